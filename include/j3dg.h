@@ -1,0 +1,187 @@
+/*
+ * j3dg.h -- C ABI of the MI355X (gfx950) ray-casting path of j3d.
+ *
+ * The reference renders a frame on the CPU inside the C++ class `canvas`
+ * (j3d/canvas.h:12-102), driven by view::render_scene (j3d/view.cpp:421-430):
+ *     update_settings(flags) -> render_scene(&scene) -> get_pixels()
+ *     -> canvas_to_image(pixels, matcap)
+ * This header is the boundary that a `canvas` implementation (or any FFI)
+ * binds to replace that path with HIP kernels on one GPU.  Plain C types
+ * only; every call returns an int status (J3DG_OK == 0); no C++ exception
+ * crosses it; j3dg_last_error() returns the message of the last failure.
+ * A context is not re-entrant: serialise calls per context, as the reference
+ * serialises render calls through view::_mut (j3d/view.cpp:1262).
+ *
+ * Matrices are float[16], column-major, translation in m[12..14]
+ * (the jtk float4x4 layout used throughout j3d/canvas.cpp).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define J3DG_ABI_VERSION 1
+
+/* status codes */
+#define J3DG_OK 0
+#define J3DG_ERR_INVALID_ARG 1
+#define J3DG_ERR_NO_DEVICE 2
+#define J3DG_ERR_OUT_OF_MEMORY 3
+#define J3DG_ERR_LAUNCH 4
+#define J3DG_ERR_NOT_FOUND 5
+#define J3DG_ERR_UNSUPPORTED 6
+#define J3DG_ERR_INTERNAL 7
+
+/* Frame flags.  Bits 0-6 mirror canvas::canvas_settings (j3d/canvas.h:16-25). */
+#define J3DG_ONE_BIT (1u << 0)
+#define J3DG_SHADOW (1u << 1)
+#define J3DG_EDGES (1u << 2)
+#define J3DG_WIREFRAME (1u << 3)
+#define J3DG_SHADING (1u << 4)
+#define J3DG_TEXTURED (1u << 5)
+#define J3DG_VERTEXCOLORS (1u << 6)
+/* Shadow rays toward the direction light_dir instead of the reference's point
+   light at pivot + 3*diagonal (j3d/canvas.cpp:703-705). */
+#define J3DG_DIRECTIONAL_LIGHT (1u << 7)
+/* render_scene(nullptr) (j3d/canvas.cpp:883-890): only db_id and object_id are reset. */
+#define J3DG_NO_SCENE (1u << 8)
+/* Use the mesh's trained tmax proposal for primary rays (exact: falls back to plain traversal). */
+#define J3DG_LEARNED (1u << 9)
+
+/* Render stages */
+#define J3DG_STAGE_TRACE 1u /* canvas::render_scene: G-buffer */
+#define J3DG_STAGE_SHADE 2u /* canvas::canvas_to_image: RGBA from the G-buffer */
+#define J3DG_STAGE_ALL 3u
+
+#define J3DG_MAX_OBJECTS 16
+
+typedef struct j3dg_ctx j3dg_ctx;
+
+/* G-buffer record, byte-identical to `pixel` (j3d/pixel.h:11-27). */
+typedef struct j3dg_pixel {
+  uint8_t mark, r, g, b;
+  float u, v, depth;
+  uint32_t object_id;
+  float barycentric_u, barycentric_v;
+  uint32_t db_id;
+} j3dg_pixel;
+
+/* Per-frame description: the state canvas::update_canvas reads
+   (j3d/canvas.cpp:677-782) plus canvas_to_image's settings. */
+typedef struct j3dg_frame_desc {
+  uint32_t width, height;       /* canvas size (im.width(), im.height()) */
+  int32_t x0, y0, x1, y1;       /* inclusive update rectangle, clamped as j3d/canvas.cpp:682-698 */
+  float proj_inv[16];           /* canvas::projection_matrix_inv */
+  float near_plane;             /* camera::nearClippingPlane (screen_pos.z) */
+  float cs[16];                 /* scene::coordinate_system */
+  float cs_inv[16];             /* scene::coordinate_system_inv */
+  float pivot[3];               /* scene::pivot */
+  float diagonal;               /* scene::diagonal */
+  uint32_t flags;               /* J3DG_* flags */
+  float light_dir[3];           /* J3DG_DIRECTIONAL_LIGHT: world direction toward the light */
+  uint32_t bg_top, bg_bottom;   /* canvas::set_background_color gradient (j3d/canvas.cpp:55-78) */
+  uint32_t n_objects;           /* scene objects in scene order (two-level index order) */
+  uint32_t objects[J3DG_MAX_OBJECTS]; /* mesh slots from j3dg_upload_mesh */
+  float margin;                 /* J3DG_LEARNED: bounded pass tmax = t_pred * (1 + margin) */
+} j3dg_frame_desc;
+
+/* BVH build options for j3dg_upload_mesh (0 = default). */
+typedef struct j3dg_build_opts {
+  uint32_t width;     /* 4 (default) or 8 children per node */
+  uint32_t leaf_size; /* max triangles per leaf, default 4 */
+  uint32_t threads;   /* host build threads, 0 = hardware_concurrency */
+} j3dg_build_opts;
+
+typedef struct j3dg_mesh_info {
+  uint64_t n_vertices, n_triangles, n_nodes;
+  uint32_t width, max_depth;
+  float bb_min[3], bb_max[3];   /* object-space bounding box (j3d/mesh.cpp:39-58) */
+  double build_seconds;         /* host BVH build + re-pack */
+  double upload_seconds;        /* host -> device copies */
+  uint64_t device_bytes;
+} j3dg_mesh_info;
+
+/* Traversal tuning (chosen by the autotuner; 0 = default). */
+typedef struct j3dg_tuning {
+  uint32_t stack_depth;    /* LDS stack entries per lane: 8, 16 or 32 */
+  uint32_t rays_per_item;  /* rays each lane takes per queue fetch: 1, 2 or 4 */
+  uint32_t persistent;     /* 1 = persistent waves on an atomic queue, 0 = one wave per tile */
+  uint32_t block_size;     /* threads per workgroup: 64..256 */
+} j3dg_tuning;
+
+/* Statistics of the last j3dg_render / j3dg_render_batch. */
+typedef struct j3dg_stats {
+  uint64_t primary_rays, shadow_rays;
+  uint64_t node_visits, tri_tests;   /* plain-kernel counters (only when counting is on) */
+  uint64_t hits, shadowed;
+  uint64_t learned_resolved;         /* primary rays resolved by the bounded pass */
+  uint64_t learned_fallback;         /* primary rays that needed the plain pass */
+  uint64_t stack_overflow;           /* must be 0 */
+  float trace_ms, shade_ms, mlp_ms;  /* HIP-event times of the last launch of each kernel */
+} j3dg_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int j3dg_create(int device, j3dg_ctx** out);
+void j3dg_destroy(j3dg_ctx* ctx);
+const char* j3dg_last_error(const j3dg_ctx* ctx);
+int j3dg_abi_version(void);
+int j3dg_device_count(int* count);
+int j3dg_synchronize(j3dg_ctx* ctx);
+int j3dg_set_tuning(j3dg_ctx* ctx, const j3dg_tuning* t);
+int j3dg_get_tuning(j3dg_ctx* ctx, j3dg_tuning* t);
+/* Per-ray node/triangle counters in the trace kernel (roofline accounting). */
+int j3dg_set_counting(j3dg_ctx* ctx, int on);
+int j3dg_get_stats(j3dg_ctx* ctx, j3dg_stats* st);
+
+/* ---- scene ------------------------------------------------------------ */
+/* Replaces scene.cpp:8-25 add_object for a mesh: computes triangle normals
+   (scene.cpp:20), the bounding box (scene.cpp:22) and the BVH (scene.cpp:23,
+   jtk::qbvh), re-packs it for the GPU and uploads it.  Host arrays are only
+   borrowed for the call.  xyz: n_vertices*3 floats; tris: n_triangles*3. */
+int j3dg_upload_mesh(j3dg_ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n_vertices, const uint32_t* tris,
+                     uint64_t n_triangles, const float cs[16], const j3dg_build_opts* opts, uint32_t* slot);
+/* scene.cpp:43-52 remove_object */
+int j3dg_remove_mesh(j3dg_ctx* ctx, uint32_t slot);
+int j3dg_set_mesh_cs(j3dg_ctx* ctx, uint32_t slot, const float cs[16]);
+int j3dg_get_mesh_info(j3dg_ctx* ctx, uint32_t slot, j3dg_mesh_info* info);
+/* Copies the per-triangle normals the upload computed (n_triangles*3 floats). */
+int j3dg_mesh_normals(j3dg_ctx* ctx, uint32_t slot, float* out);
+
+/* ---- canvas targets ----------------------------------------------------- */
+/* matcap::im (w*h 0xAABBGGRR, row-major) and matcap::cavity_clr (j3d/matcap.h:18-24). */
+int j3dg_set_matcap(j3dg_ctx* ctx, const uint32_t* im, uint32_t w, uint32_t h, uint32_t cavity);
+/* canvas::resize (j3d/canvas.cpp:117-134): a device G-buffer initialised to
+   pixel() defaults (object_id = 0xFFFFFFFF, db_id = 0, the rest 0) and an RGBA image. */
+int j3dg_create_target(j3dg_ctx* ctx, uint32_t w, uint32_t h, uint32_t* target);
+int j3dg_destroy_target(j3dg_ctx* ctx, uint32_t target);
+/* Upload a host G-buffer into the target (canvas_to_image with caller pixels). */
+int j3dg_write_target_gbuffer(j3dg_ctx* ctx, uint32_t target, const j3dg_pixel* gbuf);
+/* Device addresses of a target's buffers (for zero-copy consumers). */
+int j3dg_target_device_ptrs(j3dg_ctx* ctx, uint32_t target, void** gbuf, void** rgba);
+
+/* One frame on the target.  stages: J3DG_STAGE_*.  If gbuf_out / rgba_out are
+   non-NULL the call copies the results back and returns when they are ready;
+   otherwise it returns after enqueueing (j3dg_synchronize to wait). */
+int j3dg_render(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, uint32_t stages,
+                j3dg_pixel* gbuf_out, uint32_t* rgba_out);
+/* Copy a target back: G-buffer and/or RGBA (either may be NULL). Synchronous. */
+int j3dg_read_target(j3dg_ctx* ctx, uint32_t target, j3dg_pixel* gbuf, uint32_t* rgba);
+
+/* Many frames (cameras x meshes of uneven size) in one persistent launch,
+   captured in a HIP graph and replayed while the batch is unchanged. */
+int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n);
+
+/* ---- raw ray queries (world space; used by tests and by the learned proposal) ----
+   rays: n x 8 floats (ox,oy,oz,dx,dy,dz,t_near,t_far) in host memory.
+   Closest hit over the listed mesh slots: lexicographic min of (t, object index, triangle id). */
+int j3dg_trace_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
+                    uint32_t* obj, uint32_t* tri, float* t, float* u, float* v);
+int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
+                      uint8_t* hit);
+
+#ifdef __cplusplus
+}
+#endif

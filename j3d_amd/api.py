@@ -1,0 +1,338 @@
+"""Python host API over libj3dg.so: meshes, frames, targets, ray queries.
+
+Mirrors the reference's render sequence (j3d/view.cpp:421-430): a Renderer
+holds the device context (canvas + scene state on the GPU), `render` runs
+canvas::render_scene and canvas::canvas_to_image as HIP kernels and, on
+request, reads the G-buffer (`pixel`, j3d/pixel.h) and RGBA image back.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import ffi
+
+MESH_KEY = 1
+
+
+def make_db_id(key, index):
+    """j3d/db.h:25-28"""
+    return ((key << 29) | index) & 0xFFFFFFFF
+
+
+class J3dgError(RuntimeError):
+    pass
+
+
+def _check(lib, ctx, rc, what):
+    if rc != ffi.J3DG_OK:
+        msg = lib.j3dg_last_error(ctx) if ctx else b""
+        raise J3dgError("%s failed: %s (%s)" % (what, ffi.ERRORS.get(rc, rc), (msg or b"").decode()))
+
+
+# ---------------------------------------------------------------- camera ----
+
+def default_camera():
+    lib = ffi.load()
+    c = ffi.Camera()
+    lib.j3dg_default_camera(C.byref(c))
+    return c
+
+
+def camera_with_vertical_fov(fov_deg):
+    """Reference camera model (j3d/camera.cpp:5-16) with the focal length that
+    gives the requested vertical field of view on the 0.768-inch film."""
+    c = default_camera()
+    half_h_mm = c.film_h_inch * 25.4 / 2.0
+    c.focal_length = float(np.float32(half_h_mm / math.tan(math.radians(fov_deg) / 2.0)))
+    return c
+
+
+def projection(camera, w, h):
+    """(P, P^-1) as float32[16] (j3d/camera.cpp:26-73)."""
+    lib = ffi.load()
+    P = np.zeros(16, np.float32)
+    Pi = np.zeros(16, np.float32)
+    lib.j3dg_projection(C.byref(camera), int(w), int(h), ffi.ptr(P))
+    lib.j3dg_invert_projection(ffi.ptr(P), ffi.ptr(Pi))
+    return P, Pi
+
+
+def invert_orthonormal(m):
+    lib = ffi.load()
+    m = np.ascontiguousarray(m, np.float32)
+    o = np.zeros(16, np.float32)
+    lib.j3dg_invert_orthonormal(ffi.ptr(m), ffi.ptr(o))
+    return o
+
+
+def orbit_camera(dist, azimuth, elevation, target=(0.0, 0.0, 0.0)):
+    lib = ffi.load()
+    t = np.asarray(target, np.float32)
+    cs = np.zeros(16, np.float32)
+    ci = np.zeros(16, np.float32)
+    lib.j3dg_orbit_camera(float(dist), float(azimuth), float(elevation), ffi.ptr(t), ffi.ptr(cs), ffi.ptr(ci))
+    return cs, ci
+
+
+def translation_camera(x, y, z):
+    """Camera at (x, y, z) looking down -z (the unzoom convention, j3d/scene.cpp:91-110)."""
+    cs = np.eye(4, dtype=np.float32).reshape(16)
+    cs[12], cs[13], cs[14] = x, y, z
+    return cs, invert_orthonormal(cs)
+
+
+def make_matcap(kind=0):
+    lib = ffi.load()
+    im = np.zeros(512 * 512, np.uint32)
+    cav = C.c_uint32(0)
+    rc = lib.j3dg_make_matcap(int(kind), ffi.ptr(im), C.byref(cav))
+    if rc:
+        raise J3dgError("unknown matcap type %r" % kind)
+    return im.reshape(512, 512), int(cav.value)
+
+
+# -------------------------------------------------------------- scenegen ----
+
+def icosphere(level, seed=1, amplitude=0.05):
+    lib = ffi.load()
+    nv = lib.j3dg_icosphere_vertex_count(level)
+    nt = lib.j3dg_icosphere_triangle_count(level)
+    v = np.zeros((nv, 3), np.float32)
+    t = np.zeros((nt, 3), np.uint32)
+    rc = lib.j3dg_gen_icosphere(int(level), int(seed), float(amplitude), ffi.ptr(v), ffi.ptr(t))
+    if rc:
+        raise J3dgError("icosphere generation failed")
+    return v, t
+
+
+def torus(nu, nv, R=1.0, r=0.35, seed=1, amplitude=0.02, octaves=6, lacunarity=2.0, gain=0.5, base_freq=4.0,
+          threads=0):
+    lib = ffi.load()
+    v = np.zeros((nu * nv, 3), np.float32)
+    t = np.zeros((2 * nu * nv, 3), np.uint32)
+    rc = lib.j3dg_gen_torus(int(nu), int(nv), float(R), float(r), int(seed), float(amplitude), int(octaves),
+                            float(lacunarity), float(gain), float(base_freq), int(threads), ffi.ptr(v), ffi.ptr(t))
+    if rc:
+        raise J3dgError("torus generation failed")
+    return v, t
+
+
+def random_hemisphere(seed):
+    lib = ffi.load()
+    o = np.zeros(3, np.float32)
+    lib.j3dg_random_hemisphere(int(seed), ffi.ptr(o))
+    return o
+
+
+def bbox(v):
+    """j3d/mesh.cpp:39-58"""
+    return v.min(axis=0).astype(np.float32), v.max(axis=0).astype(np.float32)
+
+
+def scene_framing(v, cs=None):
+    """prepare_scene (j3d/scene.cpp:50-89) for one object: (diagonal, pivot)."""
+    lo, hi = bbox(v)
+    if cs is not None:
+        m = np.asarray(cs, np.float32).reshape(4, 4).T
+        a = (m[:3, :3] @ lo + m[:3, 3]).astype(np.float32)
+        b = (m[:3, :3] @ hi + m[:3, 3]).astype(np.float32)
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+    ext = hi - lo
+    diag = np.float32(max(ext[0], ext[1], ext[2]))
+    pivot = ((lo + hi) * np.float32(0.5)).astype(np.float32)
+    return float(diag), pivot
+
+
+# ------------------------------------------------------------------ frame ----
+
+class Frame:
+    """Python view of j3dg_frame_desc."""
+
+    def __init__(self, width, height, proj_inv, cs, cs_inv, pivot, diagonal, objects, flags=ffi.EDGES | ffi.SHADING,
+                 near_plane=0.1, light_dir=(0.0, 0.0, 1.0), bg_top=0xff000000, bg_bottom=0xff404040, rect=None,
+                 margin=0.0):
+        self.width, self.height = int(width), int(height)
+        self.proj_inv = np.asarray(proj_inv, np.float32)
+        self.cs = np.asarray(cs, np.float32)
+        self.cs_inv = np.asarray(cs_inv, np.float32)
+        self.pivot = np.asarray(pivot, np.float32)
+        self.diagonal = float(np.float32(diagonal))
+        self.objects = list(objects)
+        self.flags = int(flags)
+        self.near_plane = float(np.float32(near_plane))
+        self.light_dir = np.asarray(light_dir, np.float32)
+        self.bg_top, self.bg_bottom = int(bg_top), int(bg_bottom)
+        self.rect = rect if rect is not None else (0, 0, self.width - 1, self.height - 1)
+        self.margin = float(margin)
+
+    def desc(self):
+        d = ffi.FrameDesc()
+        d.width, d.height = self.width, self.height
+        d.x0, d.y0, d.x1, d.y1 = [int(x) for x in self.rect]
+        d.proj_inv[:] = [float(x) for x in self.proj_inv]
+        d.near_plane = self.near_plane
+        d.cs[:] = [float(x) for x in self.cs]
+        d.cs_inv[:] = [float(x) for x in self.cs_inv]
+        d.pivot[:] = [float(x) for x in self.pivot]
+        d.diagonal = self.diagonal
+        d.flags = self.flags
+        d.light_dir[:] = [float(x) for x in self.light_dir]
+        d.bg_top, d.bg_bottom = self.bg_top, self.bg_bottom
+        if len(self.objects) > ffi.MAX_OBJECTS:
+            raise J3dgError("at most %d objects per frame" % ffi.MAX_OBJECTS)
+        d.n_objects = len(self.objects)
+        for i, s in enumerate(self.objects):
+            d.objects[i] = int(s)
+        d.margin = self.margin
+        return d
+
+
+def make_frame(width, height, camera, cs, cs_inv, pivot, diagonal, objects, **kw):
+    _, Pi = projection(camera, width, height)
+    return Frame(width, height, Pi, cs, cs_inv, pivot, diagonal, objects, near_plane=camera.near_plane, **kw)
+
+
+# --------------------------------------------------------------- renderer ----
+
+class Renderer:
+    """One j3dg context on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = ffi.load()
+        h = C.c_void_p()
+        rc = self.lib.j3dg_create(int(device), C.byref(h))
+        _check(self.lib, None, rc, "j3dg_create")
+        self.ctx = h
+        self._targets = {}
+
+    def close(self):
+        if self.ctx:
+            self.lib.j3dg_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        _check(self.lib, self.ctx, rc, what)
+
+    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=4, leaf_size=4, threads=0):
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        o = ffi.BuildOpts(int(width), int(leaf_size), int(threads))
+        slot = C.c_uint32(0)
+        csa = None if cs is None else np.ascontiguousarray(cs, np.float32)
+        if db_id is None:
+            db_id = make_db_id(MESH_KEY, 0)
+        self._c(self.lib.j3dg_upload_mesh(self.ctx, int(db_id), ffi.ptr(v), v.shape[0], ffi.ptr(t), t.shape[0],
+                                          ffi.ptr(csa), C.byref(o), C.byref(slot)), "j3dg_upload_mesh")
+        return int(slot.value)
+
+    def remove_mesh(self, slot):
+        self._c(self.lib.j3dg_remove_mesh(self.ctx, int(slot)), "j3dg_remove_mesh")
+
+    def set_mesh_cs(self, slot, cs):
+        a = np.ascontiguousarray(cs, np.float32)
+        self._c(self.lib.j3dg_set_mesh_cs(self.ctx, int(slot), ffi.ptr(a)), "j3dg_set_mesh_cs")
+
+    def mesh_info(self, slot):
+        mi = ffi.MeshInfo()
+        self._c(self.lib.j3dg_get_mesh_info(self.ctx, int(slot), C.byref(mi)), "j3dg_get_mesh_info")
+        return {k: (list(getattr(mi, k)) if k.startswith("bb_") else getattr(mi, k)) for k, _ in mi._fields_}
+
+    def mesh_normals(self, slot, n_tris):
+        out = np.zeros((n_tris, 3), np.float32)
+        self._c(self.lib.j3dg_mesh_normals(self.ctx, int(slot), ffi.ptr(out)), "j3dg_mesh_normals")
+        return out
+
+    def set_matcap(self, im, cavity):
+        a = np.ascontiguousarray(im, np.uint32)
+        h, w = a.shape
+        self._c(self.lib.j3dg_set_matcap(self.ctx, ffi.ptr(a), w, h, int(cavity)), "j3dg_set_matcap")
+
+    def create_target(self, w, h):
+        t = C.c_uint32(0)
+        self._c(self.lib.j3dg_create_target(self.ctx, int(w), int(h), C.byref(t)), "j3dg_create_target")
+        self._targets[int(t.value)] = (int(w), int(h))
+        return int(t.value)
+
+    def destroy_target(self, t):
+        self._c(self.lib.j3dg_destroy_target(self.ctx, int(t)), "j3dg_destroy_target")
+        self._targets.pop(int(t), None)
+
+    def write_gbuffer(self, t, gbuf):
+        a = np.ascontiguousarray(gbuf, ffi.PIXEL_DTYPE)
+        self._c(self.lib.j3dg_write_target_gbuffer(self.ctx, int(t), ffi.ptr(a)), "j3dg_write_target_gbuffer")
+
+    def render(self, frame, target, stages=ffi.STAGE_ALL, readback=True):
+        d = frame.desc()
+        w, h = self._targets[int(target)]
+        if readback:
+            g = np.zeros((h, w), ffi.PIXEL_DTYPE)
+            im = np.zeros((h, w), np.uint32)
+            self._c(self.lib.j3dg_render(self.ctx, C.byref(d), int(target), int(stages), ffi.ptr(g), ffi.ptr(im)),
+                    "j3dg_render")
+            return g, im
+        self._c(self.lib.j3dg_render(self.ctx, C.byref(d), int(target), int(stages), None, None), "j3dg_render")
+        return None
+
+    def read_target(self, t):
+        w, h = self._targets[int(t)]
+        g = np.zeros((h, w), ffi.PIXEL_DTYPE)
+        im = np.zeros((h, w), np.uint32)
+        self._c(self.lib.j3dg_read_target(self.ctx, int(t), ffi.ptr(g), ffi.ptr(im)), "j3dg_read_target")
+        return g, im
+
+    def render_batch(self, frames, targets):
+        arr = (ffi.FrameDesc * len(frames))(*[f.desc() for f in frames])
+        tg = np.ascontiguousarray(targets, np.uint32)
+        self._c(self.lib.j3dg_render_batch(self.ctx, arr, ffi.ptr(tg), len(frames)), "j3dg_render_batch")
+
+    def synchronize(self):
+        self._c(self.lib.j3dg_synchronize(self.ctx), "j3dg_synchronize")
+
+    def set_tuning(self, stack_depth=16, rays_per_item=1, persistent=1, block_size=256):
+        t = ffi.Tuning(stack_depth, rays_per_item, persistent, block_size)
+        self._c(self.lib.j3dg_set_tuning(self.ctx, C.byref(t)), "j3dg_set_tuning")
+
+    def get_tuning(self):
+        t = ffi.Tuning()
+        self._c(self.lib.j3dg_get_tuning(self.ctx, C.byref(t)), "j3dg_get_tuning")
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_counting(self, on):
+        self._c(self.lib.j3dg_set_counting(self.ctx, 1 if on else 0), "j3dg_set_counting")
+
+    def stats(self):
+        s = ffi.Stats()
+        self._c(self.lib.j3dg_get_stats(self.ctx, C.byref(s)), "j3dg_get_stats")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def trace_rays(self, slots, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        s = np.ascontiguousarray(slots, np.uint32)
+        out = {"obj": np.zeros(n, np.uint32), "tri": np.zeros(n, np.uint32), "t": np.zeros(n, np.float32),
+               "u": np.zeros(n, np.float32), "v": np.zeros(n, np.float32)}
+        self._c(self.lib.j3dg_trace_rays(self.ctx, ffi.ptr(s), len(s), ffi.ptr(r), n, ffi.ptr(out["obj"]),
+                                         ffi.ptr(out["tri"]), ffi.ptr(out["t"]), ffi.ptr(out["u"]),
+                                         ffi.ptr(out["v"])), "j3dg_trace_rays")
+        return out
+
+    def any_hit(self, slots, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        s = np.ascontiguousarray(slots, np.uint32)
+        hit = np.zeros(r.shape[0], np.uint8)
+        self._c(self.lib.j3dg_any_hit_rays(self.ctx, ffi.ptr(s), len(s), ffi.ptr(r), r.shape[0], ffi.ptr(hit)),
+                "j3dg_any_hit_rays")
+        return hit

@@ -1,0 +1,199 @@
+// j3dg_internal.h -- context, device records and per-frame parameters of the
+// gfx950 ray-casting library (private to j3d_amd/csrc).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "j3dg.h"
+#include "rt_math.h"
+
+namespace j3d {
+
+// Device record of one scene object (scene_object, j3d/scene.h:10-25).
+struct DevObject {
+  const float4* nodes;     // wide BVH nodes
+  const float4* tris;      // triangle records in leaf order
+  const float* verts;      // original vertices (shadow-ray origins)
+  const uint32_t* idx;     // original triangles
+  const float* normals;    // per-triangle normals
+  float cs[16];            // object -> world
+  float inv_cs[16];        // world -> object
+  uint32_t db_id;
+  uint32_t width;
+  uint32_t pad[2];
+};
+
+// Frame parameters of the trace kernel (passed by value).
+struct FrameParams {
+  uint32_t w, h;
+  int32_t x0, y0, x1, y1;   // clamped, inclusive
+  uint32_t tiles_x, n_tiles;
+  float proj_inv[16];
+  float near_plane;
+  float cs[16], cs_inv[16];
+  float diagonal;
+  float light[4];           // point light, world (j3d/canvas.cpp:703-705)
+  float light_dir[3];
+  uint32_t flags;
+  uint32_t n_obj;
+  j3dg_pixel* gbuf;
+  const float2* proposal;   // learned: per-pixel (t_pred, hit logit), rect-indexed
+  float margin;
+};
+
+struct ShadeParams {
+  uint32_t w, h;
+  const j3dg_pixel* gbuf;
+  uint32_t* rgba;
+  const uint32_t* matcap;
+  uint32_t mw, mh, cavity;
+  uint32_t flags;
+  float proj_inv[16];
+  float near_plane;
+  uint32_t bg_top, bg_bottom;
+};
+
+inline void invert_orthonormal(const float* m, float* o) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o[4 * j + i] = m[4 * i + j];
+  o[3] = o[7] = o[11] = 0.f;
+  for (int i = 0; i < 3; ++i) o[12 + i] = -(m[4 * i] * m[12] + m[4 * i + 1] * m[13] + m[4 * i + 2] * m[14]);
+  o[15] = 1.f;
+}
+
+struct Mesh {
+  bool live = false;
+  uint32_t db_id = 0;
+  float cs[16], inv_cs[16];
+  j3dg_mesh_info info{};
+  uint32_t max_stack = 1;
+  float4* d_nodes = nullptr;
+  float4* d_tris = nullptr;
+  float* d_verts = nullptr;
+  uint32_t* d_idx = nullptr;
+  float* d_normals = nullptr;
+  std::vector<float> normals_host;
+  // learned proposal (weights on the device, see mlp.hip)
+  void* d_mlp = nullptr;
+  bool has_proposal = false;
+  void release() {
+    if (d_nodes) hipFree(d_nodes);
+    if (d_tris) hipFree(d_tris);
+    if (d_verts) hipFree(d_verts);
+    if (d_idx) hipFree(d_idx);
+    if (d_normals) hipFree(d_normals);
+    if (d_mlp) hipFree(d_mlp);
+    d_nodes = nullptr, d_tris = nullptr, d_verts = nullptr, d_idx = nullptr, d_normals = nullptr, d_mlp = nullptr;
+    normals_host.clear();
+    has_proposal = false;
+    live = false;
+  }
+};
+
+struct Target {
+  bool live = false;
+  uint32_t w = 0, h = 0;
+  j3dg_pixel* gbuf = nullptr;
+  uint32_t* rgba = nullptr;
+  void release() {
+    if (gbuf) hipFree(gbuf);
+    if (rgba) hipFree(rgba);
+    gbuf = nullptr, rgba = nullptr;
+    live = false;
+  }
+};
+
+struct Ctx {
+  std::mutex mu;
+  int device = 0;
+  int n_cus = 1;
+  std::string arch;
+  std::string last_error;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool traced = false, shaded = false, mlp_ran = false;
+  bool counting = false;
+  j3dg_tuning tuning{16, 1, 1, 256};
+  std::vector<Mesh> meshes;
+  std::vector<Target> targets;
+  uint32_t* d_queue = nullptr;
+  uint64_t* d_counters = nullptr;
+  uint2* spill = nullptr;
+  uint64_t spill_entries = 0;
+  DevObject* d_objs = nullptr;
+  uint32_t obj_capacity = 0;
+  std::vector<uint8_t> obj_cache;
+  uint32_t* d_matcap = nullptr;
+  uint64_t matcap_capacity = 0;
+  uint32_t matcap_w = 0, matcap_h = 0, cavity = 0;
+
+  int fail(int code, const std::string& msg);
+  int ensure_spill(uint64_t lanes, uint32_t entries);
+  int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
+  int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned);
+  int launch_shade(const ShadeParams& sp);
+  void invalidate_graphs() {}
+  void release_buffers() {
+    if (d_queue) hipFree(d_queue);
+    if (d_counters) hipFree(d_counters);
+    if (spill) hipFree(spill);
+    if (d_objs) hipFree(d_objs);
+    if (d_matcap) hipFree(d_matcap);
+    d_queue = nullptr, d_counters = nullptr, spill = nullptr, d_objs = nullptr, d_matcap = nullptr;
+  }
+};
+
+// canvas::update_canvas's frame set-up (j3d/canvas.cpp:679-705) on the host.
+inline int make_frame_params(const j3dg_frame_desc& fd, const Target& tg, FrameParams& fp) {
+  std::memset(&fp, 0, sizeof(fp));
+  const uint32_t w = fd.width, h = fd.height;
+  if (w == 0 || h == 0) return J3DG_ERR_INVALID_ARG;
+  int x0 = fd.x0, y0 = fd.y0, x1 = fd.x1, y1 = fd.y1;
+  if (x0 < 0) x0 = 0;
+  if (y0 < 0) y0 = 0;
+  if (x1 < 0) x1 = 0;
+  if (y1 < 0) y1 = 0;
+  if (x0 >= (int)w) x0 = w - 1;
+  if (y0 >= (int)h) y0 = h - 1;
+  if (x1 >= (int)w) x1 = w - 1;
+  if (y1 >= (int)h) y1 = h - 1;
+  fp.w = w, fp.h = h;
+  fp.x0 = x0, fp.y0 = y0, fp.x1 = x1, fp.y1 = y1;
+  // an inverted rectangle renders nothing (the reference's loops do not run)
+  uint32_t rw = x1 >= x0 ? (uint32_t)(x1 - x0 + 1) : 0, rh = y1 >= y0 ? (uint32_t)(y1 - y0 + 1) : 0;
+  fp.tiles_x = (rw + 7) / 8;
+  fp.n_tiles = fp.tiles_x * ((rh + 7) / 8);
+  std::memcpy(fp.proj_inv, fd.proj_inv, sizeof(fp.proj_inv));
+  fp.near_plane = fd.near_plane;
+  std::memcpy(fp.cs, fd.cs, sizeof(fp.cs));
+  std::memcpy(fp.cs_inv, fd.cs_inv, sizeof(fp.cs_inv));
+  fp.diagonal = fd.diagonal;
+  f4 light = {fd.pivot[0] + fd.diagonal * 3.f, fd.pivot[1] + fd.diagonal * 3.f, fd.pivot[2] + fd.diagonal * 3.f, 1.f};
+  light = mat_vec(fd.cs, light);
+  fp.light[0] = light.x, fp.light[1] = light.y, fp.light[2] = light.z, fp.light[3] = light.w;
+  fp.light_dir[0] = fd.light_dir[0], fp.light_dir[1] = fd.light_dir[1], fp.light_dir[2] = fd.light_dir[2];
+  fp.flags = fd.flags;
+  fp.n_obj = fd.n_objects;
+  fp.gbuf = tg.gbuf;
+  fp.margin = fd.margin;
+  return J3DG_OK;
+}
+
+inline void make_shade_params(const j3dg_frame_desc& fd, const Target& tg, const Ctx& ctx, ShadeParams& sp) {
+  std::memset(&sp, 0, sizeof(sp));
+  sp.w = tg.w, sp.h = tg.h;
+  sp.gbuf = tg.gbuf;
+  sp.rgba = tg.rgba;
+  sp.matcap = ctx.d_matcap;
+  sp.mw = ctx.matcap_w, sp.mh = ctx.matcap_h, sp.cavity = ctx.cavity;
+  sp.flags = fd.flags;
+  std::memcpy(sp.proj_inv, fd.proj_inv, sizeof(sp.proj_inv));
+  sp.near_plane = fd.near_plane;
+  sp.bg_top = fd.bg_top, sp.bg_bottom = fd.bg_bottom;
+}
+
+}  // namespace j3d

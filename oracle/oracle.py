@@ -1,0 +1,257 @@
+"""ctypes wrapper of oracle/liboracle.so -- the CPU restatement of j3d's
+ray-casting path (see oracle.h for what it restates and how it is pinned).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ONE_BIT = 1 << 0
+SHADOW = 1 << 1
+EDGES = 1 << 2
+WIREFRAME = 1 << 3
+SHADING = 1 << 4
+TEXTURED = 1 << 5
+VERTEXCOLORS = 1 << 6
+DIRECTIONAL_LIGHT = 1 << 7
+
+PIXEL_DTYPE = np.dtype([("mark", "u1"), ("r", "u1"), ("g", "u1"), ("b", "u1"), ("u", "<f4"), ("v", "<f4"),
+                        ("depth", "<f4"), ("object_id", "<u4"), ("barycentric_u", "<f4"),
+                        ("barycentric_v", "<f4"), ("db_id", "<u4")])
+
+
+class Camera(C.Structure):
+    _fields_ = [("focal_length", C.c_float), ("film_w_inch", C.c_float), ("film_h_inch", C.c_float),
+                ("near_plane", C.c_float), ("far_plane", C.c_float), ("zoom", C.c_float), ("fit_film", C.c_int32)]
+
+
+class Object(C.Structure):
+    _fields_ = [("bvh", C.c_void_p), ("vertices", C.c_void_p), ("triangles", C.c_void_p), ("normals", C.c_void_p),
+                ("n_triangles", C.c_uint64), ("cs", C.c_float * 16), ("db_id", C.c_uint32)]
+
+
+class FrameC(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("x0", C.c_int32), ("y0", C.c_int32), ("x1", C.c_int32),
+                ("y1", C.c_int32), ("proj_inv", C.c_float * 16), ("near_plane", C.c_float),
+                ("cs", C.c_float * 16), ("cs_inv", C.c_float * 16), ("pivot", C.c_float * 3),
+                ("diagonal", C.c_float), ("flags", C.c_uint32), ("light_dir", C.c_float * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("hits", C.c_uint64),
+                ("shadowed", C.c_uint64), ("near_ties", C.c_uint64)]
+
+
+_P = C.c_void_p
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_default_camera": (None, [C.POINTER(Camera)]),
+        "orc_projection": (None, [C.POINTER(Camera), C.c_int, C.c_int, _P]),
+        "orc_invert_projection": (None, [_P, _P]),
+        "orc_invert_orthonormal": (None, [_P, _P]),
+        "orc_mat_vec": (None, [_P, _P, _P]),
+        "orc_matcap": (None, [C.c_int, _P, C.POINTER(C.c_uint32)]),
+        "orc_fill_background": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "orc_triangle_normals": (None, [_P, _P, C.c_uint64, _P]),
+        "orc_bbox": (None, [_P, C.c_uint64, _P, _P]),
+        "orc_prepare_and_unzoom": (None, [C.c_uint32, _P, _P, _P, _P, _P, _P, _P]),
+        "orc_bvh_build": (_P, [_P, C.c_uint64, _P, C.c_uint64, C.c_int]),
+        "orc_bvh_free": (None, [_P]),
+        "orc_bvh_node_count": (C.c_uint64, [_P]),
+        "orc_update_canvas": (C.c_int, [_P, C.c_uint32, C.POINTER(FrameC), _P, C.c_int, C.c_int, C.POINTER(Stats)]),
+        "orc_canvas_to_image": (None, [_P, C.c_uint32, C.c_uint32, _P, C.c_float, C.c_uint32, _P, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, _P]),
+        "orc_closest_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P, _P, _P, _P, _P, _P]),
+        "orc_any_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P]),
+        "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_camera():
+    c = Camera()
+    load().orc_default_camera(C.byref(c))
+    return c
+
+
+def projection(camera, w, h):
+    P = np.zeros(16, np.float32)
+    Pi = np.zeros(16, np.float32)
+    lib = load()
+    lib.orc_projection(C.byref(camera), int(w), int(h), _p(P))
+    lib.orc_invert_projection(_p(P), _p(Pi))
+    return P, Pi
+
+
+def invert_orthonormal(m):
+    m = np.ascontiguousarray(m, np.float32)
+    o = np.zeros(16, np.float32)
+    load().orc_invert_orthonormal(_p(m), _p(o))
+    return o
+
+
+def mat_vec(m, v):
+    m = np.ascontiguousarray(m, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    o = np.zeros(4, np.float32)
+    load().orc_mat_vec(_p(m), _p(v), _p(o))
+    return o
+
+
+def matcap(kind=0):
+    im = np.zeros(512 * 512, np.uint32)
+    cav = C.c_uint32()
+    load().orc_matcap(int(kind), _p(im), C.byref(cav))
+    return im.reshape(512, 512), int(cav.value)
+
+
+def background(w, h, top=0xff000000, bottom=0xff404040):
+    im = np.zeros((h, w), np.uint32)
+    load().orc_fill_background(_p(im), w, h, top, bottom)
+    return im
+
+
+def triangle_normals(v, t):
+    v = np.ascontiguousarray(v, np.float32)
+    t = np.ascontiguousarray(t, np.uint32)
+    n = np.zeros((t.shape[0], 3), np.float32)
+    load().orc_triangle_normals(_p(v), _p(t), t.shape[0], _p(n))
+    return n
+
+
+class Mesh:
+    """One scene object for the oracle: arrays, normals and (optionally) a BVH."""
+
+    def __init__(self, verts, tris, cs=None, db_id=(1 << 29), use_bvh=True, threads=0):
+        self.v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        self.t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        self.n = triangle_normals(self.v, self.t)
+        self.cs = np.eye(4, dtype=np.float32).reshape(16) if cs is None else np.asarray(cs, np.float32)
+        self.db_id = int(db_id)
+        self.bvh = None
+        if use_bvh:
+            self.bvh = load().orc_bvh_build(_p(self.v), self.v.shape[0], _p(self.t), self.t.shape[0], int(threads))
+
+    def obj(self, brute=False):
+        o = Object()
+        o.bvh = None if brute else self.bvh
+        o.vertices = self.v.ctypes.data
+        o.triangles = self.t.ctypes.data
+        o.normals = self.n.ctypes.data
+        o.n_triangles = self.t.shape[0]
+        o.cs[:] = [float(x) for x in self.cs]
+        o.db_id = self.db_id
+        return o
+
+    def __del__(self):
+        if getattr(self, "bvh", None):
+            load().orc_bvh_free(self.bvh)
+            self.bvh = None
+
+
+def _objs(meshes, brute=False):
+    arr = (Object * max(1, len(meshes)))(*[m.obj(brute) for m in meshes])
+    return arr, len(meshes)
+
+
+def update_canvas(meshes, frame, gbuf=None, threads=0, count_ties=False, brute=False):
+    """canvas::update_canvas on a G-buffer (created with pixel() defaults if None).
+    `frame` is a j3d_amd.Frame-like object (width, height, rect, proj_inv, ...)."""
+    w, h = frame.width, frame.height
+    if gbuf is None:
+        gbuf = np.zeros((h, w), PIXEL_DTYPE)
+        gbuf["object_id"] = 0xFFFFFFFF
+    f = FrameC()
+    f.w, f.h = w, h
+    f.x0, f.y0, f.x1, f.y1 = [int(x) for x in frame.rect]
+    f.proj_inv[:] = [float(x) for x in frame.proj_inv]
+    f.near_plane = float(frame.near_plane)
+    f.cs[:] = [float(x) for x in frame.cs]
+    f.cs_inv[:] = [float(x) for x in frame.cs_inv]
+    f.pivot[:] = [float(x) for x in frame.pivot]
+    f.diagonal = float(frame.diagonal)
+    f.flags = int(frame.flags) & 0xFF
+    f.light_dir[:] = [float(x) for x in frame.light_dir]
+    arr, n = _objs(meshes, brute)
+    st = Stats()
+    load().orc_update_canvas(arr, n, C.byref(f), _p(gbuf), int(threads), 1 if count_ties else 0, C.byref(st))
+    return gbuf, {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def canvas_to_image(gbuf, frame, matcap_im, cavity, im=None):
+    h, w = gbuf.shape
+    if im is None:
+        im = background(w, h, frame.bg_top, frame.bg_bottom)
+    pinv = np.ascontiguousarray(frame.proj_inv, np.float32)
+    mi = np.ascontiguousarray(matcap_im, np.uint32)
+    load().orc_canvas_to_image(_p(np.ascontiguousarray(gbuf)), w, h, _p(pinv), float(frame.near_plane),
+                               int(frame.flags) & 0xFF, _p(mi), mi.shape[1], mi.shape[0], int(cavity), _p(im))
+    return im
+
+
+def render(meshes, frame, matcap_im, cavity, threads=0, count_ties=False, gbuf=None):
+    g, st = update_canvas(meshes, frame, gbuf=gbuf, threads=threads, count_ties=count_ties)
+    im = canvas_to_image(g, frame, matcap_im, cavity)
+    return g, im, st
+
+
+def closest_hits(meshes, rays, threads=0, brute=False, second=False):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = r.shape[0]
+    out = {"obj": np.zeros(n, np.uint32), "tri": np.zeros(n, np.uint32), "t": np.zeros(n, np.float32),
+           "u": np.zeros(n, np.float32), "v": np.zeros(n, np.float32)}
+    t2 = np.zeros(n, np.float32) if second else None
+    arr, no = _objs(meshes, brute)
+    load().orc_closest_hits(arr, no, _p(r), n, int(threads), _p(out["obj"]), _p(out["tri"]), _p(out["t"]),
+                            _p(out["u"]), _p(out["v"]), _p(t2))
+    if second:
+        out["t2"] = t2
+    return out
+
+
+def any_hits(meshes, rays, threads=0, brute=False):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hit = np.zeros(r.shape[0], np.uint8)
+    arr, no = _objs(meshes, brute)
+    load().orc_any_hits(arr, no, _p(r), r.shape[0], int(threads), _p(hit))
+    return hit
+
+
+def intersect_triangle(o, d, v0, v1, v2, t_near=0.0, t_far=3.4028234663852886e38):
+    a = [np.ascontiguousarray(x, np.float32) for x in (o, d, v0, v1, v2)]
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = load().orc_intersect_triangle(*[_p(x) for x in a], float(t_near), float(t_far), C.byref(t), C.byref(u),
+                                        C.byref(v))
+    return bool(hit), t.value, u.value, v.value

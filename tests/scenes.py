@@ -1,0 +1,72 @@
+"""Workloads shared by the tests and the benchmark, with the comparison rules
+used for parity against the CPU oracle."""
+import math
+
+import numpy as np
+
+import j3d_amd as J
+from j3d_amd import ffi
+
+MISS = 0xFFFFFFFF
+
+
+def c0_scene(level=7, w=1024, h=768, seed=1):
+    """Seeded icosphere (level 7: 163,842 vertices, 327,680 triangles), camera at
+    (0,0,3) looking at the origin, 60 degree vertical fov, edges + shading."""
+    v, t = J.icosphere(level, seed=seed, amplitude=0.05)
+    diag, pivot = J.scene_framing(v)
+    cs, ci = J.translation_camera(0.0, 0.0, 3.0)
+    cam = J.camera_with_vertical_fov(60.0)
+    return v, t, cam, cs, ci, pivot, diag
+
+
+def torus_scene(nu, nv, seed=1, azimuth_seed=11):
+    """Seeded displaced torus (R=1, r=0.35, fBm 6 octaves amplitude 0.02,
+    lacunarity 2, gain 0.5) and an orbit camera at distance 3, 60 degree fov."""
+    v, t = J.torus(nu, nv, seed=seed)
+    diag, pivot = J.scene_framing(v)
+    st = np.array([azimuth_seed], np.uint64)
+    lib = ffi.load()
+    import ctypes as C
+    s = C.c_uint64(azimuth_seed)
+    az = 2.0 * math.pi * lib.j3dg_uniform01(C.byref(s))
+    el = math.radians(20.0 + 30.0 * lib.j3dg_uniform01(C.byref(s)))
+    del st
+    cs, ci = J.orbit_camera(3.0, az, el)
+    cam = J.camera_with_vertical_fov(60.0)
+    light = J.random_hemisphere(azimuth_seed + 1)
+    return v, t, cam, cs, ci, pivot, diag, light
+
+
+def compare_gbuffers(g, og):
+    """Bit-exact comparison of every G-buffer field.  Returns a dict of mismatch counts."""
+    out = {}
+    for k in ("mark", "r", "g", "b", "object_id", "db_id"):
+        out[k] = int((g[k] != og[k]).sum())
+    for k in ("u", "v", "depth", "barycentric_u", "barycentric_v"):
+        out[k] = int((g[k].view(np.uint32) != og[k].view(np.uint32)).sum())
+    return out
+
+
+def compare_rgba(im, oim):
+    a = im.view(np.uint8).astype(np.int16)
+    b = oim.view(np.uint8).astype(np.int16)
+    d = np.abs(a - b)
+    return int(d.max()) if d.size else 0, int((d.reshape(im.shape + (4,)).max(axis=-1) > 0).sum())
+
+
+def random_rays(n, seed, center, radius, t_near=1e-3):
+    """Rays from a sphere of radius 2*radius around center toward random points
+    near the object (world space, 8 floats each)."""
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, 3))
+    o /= np.linalg.norm(o, axis=1, keepdims=True)
+    o = center + 2.0 * radius * o
+    tgt = center + radius * rng.uniform(-1, 1, size=(n, 3))
+    d = tgt - o
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3:6] = d
+    r[:, 6] = t_near
+    r[:, 7] = np.finfo(np.float32).max
+    return r

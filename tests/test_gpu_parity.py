@@ -1,0 +1,87 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: every G-buffer field bit-exact (hit ids, db ids, depth, normals,
+barycentrics, marks); RGBA within 1 LSB per channel (acos in the edge pass
+may differ by an ulp between the device and host math libraries).
+"""
+import numpy as np
+import pytest
+
+import j3d_amd as J
+import oracle as O
+from j3d_amd import ffi
+from scenes import MISS, c0_scene, compare_gbuffers, compare_rgba, random_rays, torus_scene
+
+pytestmark = pytest.mark.gpu
+
+EDGES = ffi.EDGES | ffi.SHADING
+
+
+def _render_both(r, meshes_gpu, meshes_orc, frame, matcap=0):
+    mc, cav = J.make_matcap(matcap)
+    r.set_matcap(mc, cav)
+    tg = r.create_target(frame.width, frame.height)
+    try:
+        g, im = r.render(frame, tg)
+    finally:
+        r.destroy_target(tg)
+    og, oim, st = O.render(meshes_orc, frame, mc, cav)
+    return g, im, og, oim, st
+
+
+def _assert_parity(g, im, og, oim, label):
+    mm = compare_gbuffers(g, og)
+    assert sum(mm.values()) == 0, "%s: G-buffer mismatches %s" % (label, mm)
+    dmax, ndiff = compare_rgba(im, oim)
+    print("%s: %d hit pixels, RGBA max diff %d on %d pixels" % (label, int((g["object_id"] != MISS).sum()), dmax,
+                                                                ndiff))
+    assert dmax <= 1, "%s: RGBA differs by %d" % (label, dmax)
+
+
+def test_c0_full_size(renderer):
+    v, t, cam, cs, ci, pivot, diag = c0_scene()
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    fr = J.make_frame(1024, 768, cam, cs, ci, pivot, diag, [slot], flags=EDGES)
+    g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr)
+    _assert_parity(g, im, og, oim, "C0 1024x768")
+    assert st["hits"] > 100000
+    renderer.remove_mesh(slot)
+
+
+@pytest.mark.parametrize("flags", [EDGES | ffi.SHADOW, EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT, ffi.SHADING,
+                                   ffi.WIREFRAME | ffi.SHADING, ffi.ONE_BIT])
+def test_modes_icosphere(renderer, flags):
+    v, t, cam, cs, ci, pivot, diag = c0_scene(level=4)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 3))
+    fr = J.make_frame(320, 240, cam, cs, ci, pivot, diag, [slot], flags=flags,
+                      light_dir=J.random_hemisphere(5))
+    g, im, og, oim, _ = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(1, 3))], fr)
+    _assert_parity(g, im, og, oim, "icosphere L4 flags=%#x" % flags)
+    renderer.remove_mesh(slot)
+
+
+def test_torus_shadow_directional(renderer):
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(400, 160)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    fr = J.make_frame(640, 360, cam, cs, ci, pivot, diag, [slot],
+                      flags=EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT, light_dir=light)
+    g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr)
+    _assert_parity(g, im, og, oim, "torus 128k tris 640x360 shadow")
+    assert st["shadowed"] > 0
+    renderer.remove_mesh(slot)
+
+
+def test_random_rays_closest_and_any(renderer):
+    v, t = J.torus(300, 120, seed=3)
+    slot = renderer.upload_mesh(v, t)
+    rays = random_rays(200000, 9, np.zeros(3), 1.4)
+    got = renderer.trace_rays([slot], rays)
+    m = O.Mesh(v, t)
+    ref = O.closest_hits([m], rays)
+    for k in ("obj", "tri"):
+        assert np.array_equal(got[k], ref[k]), k
+    for k in ("t", "u", "v"):
+        assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), k
+    hit = renderer.any_hit([slot], rays)
+    assert np.array_equal(hit, O.any_hits([m], rays))
+    renderer.remove_mesh(slot)
