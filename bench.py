@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Ray-casting benchmark of j3d's per-frame path on one MI355X.
+
+Workload (default): seeded displaced torus, 5000 x 2800 grid = 14,000,000
+vertices / 28,000,000 triangles; orbit camera at distance 3, 60 degree vertical
+fov, 1920 x 1080 primary rays plus one directional shadow ray per hit pixel;
+edges + matcap shading.  A step is one frame: raygen + BVH traversal (+ the
+learned proposal when enabled) + shadow rays + G-buffer write (k_trace) and the
+shading/edge pass (k_shade), with the mesh and BVH already resident in HBM.
+
+value = (primary + shadow rays) / GPU time per frame, in Mrays/s.
+
+The JSON line also carries:
+  roofline      the trace kernel's algorithmic bytes per launch (node visits x
+                node bytes + triangle tests x 48 B + 32 B G-buffer write per
+                pixel, from the kernel's own per-ray counters) over its mean
+                HIP-event duration, against 8 TB/s; `traffic` is the HBM bytes
+                per launch measured by rocprofv3 (profiles/), when present.
+  cpu_baseline  the CPU oracle renderer (the project's restatement of
+                canvas::update_canvas + canvas_to_image with its own BVH and a
+                row-parallel std::thread pool) timed on this host's cores on a
+                band of rows of the same frame.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s @28M-tri 1080p: learned-bounded vs plain GPU traversal; % HBM peak"
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k"])
+    ap.add_argument("--width", type=int, default=4, choices=[4, 8])
+    ap.add_argument("--stack-depth", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=-1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows in the CPU baseline band (0 = auto)")
+    ap.add_argument("--profile-only", action="store_true", help="no CPU baseline, short output (for rocprofv3)")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    import j3d_amd as J
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scenes import c0_scene, torus_scene
+    from j3d_amd import ffi
+    if name == "torus28m":
+        t0 = time.time()
+        v, t, cam, cs, ci, pivot, diag, light = torus_scene(5000, 2800, seed=1)
+        log("torus 5000x2800: %d verts %d tris in %.1fs" % (len(v), len(t), time.time() - t0))
+        w, h = 1920, 1080
+        flags = ffi.EDGES | ffi.SHADING | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+        desc = {"workload": "displaced torus R=1 r=0.35, 5000x2800 grid, 28,000,000 triangles, fBm 6 octaves; "
+                            "1920x1080 primary + directional shadow rays, edges + matcap",
+                "triangles": int(len(t)), "width": w, "height": h, "shadow": "directional"}
+    else:
+        v, t, cam, cs, ci, pivot, diag = c0_scene()
+        light = np.array([0, 0, 1], np.float32)
+        w, h = 1024, 768
+        flags = ffi.EDGES | ffi.SHADING
+        desc = {"workload": "displaced icosphere level 7, 327,680 triangles; 1024x768 primary rays, edges + matcap",
+                "triangles": int(len(t)), "width": w, "height": h, "shadow": "off"}
+    return v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc
+
+
+def read_traffic(workload_name):
+    """HBM bytes per trace launch measured by rocprofv3 --pmc (committed summary)."""
+    p = os.path.join(ROOT, "profiles", "trace_%s_traffic.json" % workload_name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, rows):
+    """Times the oracle's update_canvas + canvas_to_image on a band of rows."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    t0 = time.time()
+    m = O.Mesh(v, t, db_id=(1 << 29), threads=threads)
+    bvh_s = time.time() - t0
+    h, w = frame.height, frame.width
+    y0 = max(0, h // 2 - rows // 2)
+    y1 = min(h - 1, y0 + rows - 1)
+    import copy
+    fr = copy.copy(frame)
+    fr.rect = (0, y0, w - 1, y1)
+    g = np.zeros((h, w), O.PIXEL_DTYPE)
+    g["object_id"] = 0xFFFFFFFF
+    t0 = time.time()
+    g, st = O.update_canvas([m], fr, gbuf=g, threads=threads)
+    im = O.canvas_to_image(g, fr, mc, cav)
+    dt = time.time() - t0
+    rays = st["primary_rays"] + st["shadow_rays"]
+    band = slice(y0, y1 + 1)
+    mism = int((g["object_id"][band] != gpu_gbuf["object_id"][band]).sum() +
+               (g["depth"][band].view(np.uint32) != gpu_gbuf["depth"][band].view(np.uint32)).sum() +
+               (g["mark"][band] != gpu_gbuf["mark"][band]).sum())
+    # edge pixels of the band's first row see an up neighbour outside the band
+    d = np.abs(im[y0 + 1:y1 + 1].view(np.uint8).astype(int) - gpu_im[y0 + 1:y1 + 1].view(np.uint8).astype(int))
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "rows %d-%d of the %dx%d frame: %d primary + %d shadow rays, update_canvas + canvas_to_image "
+                      "in %.2fs (oracle BVH build %.1fs excluded)" % (y0, y1, w, h, st["primary_rays"],
+                                                                       st["shadow_rays"], dt, bvh_s),
+            "cpu_model": model, "gbuffer_mismatches_vs_gpu": mism, "rgba_max_diff_vs_gpu": int(d.max())}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by image "
+              "tiles or cameras with a final RGBA gather); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import j3d_amd as J
+    from j3d_amd import ffi
+    v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc = workload(a.workload)
+    mc, cav = J.make_matcap(0)
+    r = J.Renderer(0)
+    tun = r.get_tuning()
+    if a.stack_depth:
+        tun["stack_depth"] = a.stack_depth
+    if a.block:
+        tun["block_size"] = a.block
+    if a.persistent >= 0:
+        tun["persistent"] = a.persistent
+    r.set_tuning(**tun)
+    t0 = time.time()
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width)
+    info = r.mesh_info(slot)
+    log("upload: build %.2fs copy %.2fs, %d nodes, depth %d, %.2f GB on device" % (
+        info["build_seconds"], info["upload_seconds"], info["n_nodes"], info["max_depth"],
+        info["device_bytes"] / 1e9))
+    r.set_matcap(mc, cav)
+    tg = r.create_target(w, h)
+    fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+
+    # counters (node visits, triangle tests) from one counting frame
+    r.set_counting(True)
+    gpu_g, gpu_im = r.render(fr, tg)
+    st = r.stats()
+    r.set_counting(False)
+    assert st["stack_overflow"] == 0
+    rays_per_frame = st["primary_rays"] + st["shadow_rays"]
+
+    for _ in range(a.warmup):
+        r.render(fr, tg, readback=False)
+    r.synchronize()
+    r.reset_timings()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render(fr, tg, readback=False)
+    r.synchronize()
+    wall = time.perf_counter() - t0
+    tm = r.timings()
+    trace_ms = float(np.mean(tm["trace_ms"]))
+    shadow_ms = float(np.mean(tm["shadow_ms"]))
+    shade_ms = float(np.mean(tm["shade_ms"]))
+    ms_per_step = wall / a.steps * 1e3
+    value = rays_per_frame * a.steps / wall / 1e6
+
+    node_bytes = 32 * a.width
+    npx = w * h
+    # primary k_trace launch: node and triangle reads + 20 B of G-buffer writes per pixel
+    algo_bytes = st["node_visits"] * node_bytes + st["tri_tests"] * 48 + 20 * npx
+    achieved = algo_bytes / (trace_ms * 1e-3)
+    traffic = read_traffic(a.workload)
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": (traffic or {}).get("hbm_bytes_per_launch") if traffic else None,
+            "kernel": "k_trace", "launch_ms": trace_ms,
+            "algorithmic_bytes_per_launch": int(algo_bytes),
+            "per_unit": "%d B per node visit, 48 B per triangle test, 20 B per pixel" % node_bytes}
+
+    out = {"metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
+           "roofline": roof,
+           "plain_mrays": value,
+           "primary_mrays": st["primary_rays"] * a.steps / wall / 1e6,
+           "rays_per_frame": int(rays_per_frame), "primary_rays": int(st["primary_rays"]),
+           "shadow_rays": int(st["shadow_rays"]), "hit_pixels": int(st["hits"]),
+           "node_visits_per_ray": st["node_visits"] / max(1, rays_per_frame),
+           "tri_tests_per_ray": st["tri_tests"] / max(1, rays_per_frame),
+           "trace_ms": trace_ms, "shadow_stage_ms": shadow_ms, "shade_ms": shade_ms,
+           "shadow_node_visits_per_ray": st["shadow_node_visits"] / max(1, st["shadow_rays"]),
+           "bvh": {"width": a.width, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
+                   "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
+           "tuning": r.get_tuning()}
+
+    if not a.no_cpu_baseline and not a.profile_only:
+        rows = a.cpu_rows or max(8, min(h, 216))
+        log("cpu baseline on %d rows ..." % rows)
+        out["cpu_baseline"] = cpu_baseline(v, t, fr, mc, cav, gpu_g, gpu_im, rows)
+    else:
+        out["cpu_baseline"] = None
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

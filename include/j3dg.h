@@ -108,8 +108,9 @@ typedef struct j3dg_mesh_info {
 typedef struct j3dg_tuning {
   uint32_t stack_depth;    /* LDS stack entries per lane: 8, 16 or 32 */
   uint32_t rays_per_item;  /* rays each lane takes per queue fetch: 1, 2 or 4 */
-  uint32_t persistent;     /* 1 = persistent waves on an atomic queue, 0 = one wave per tile */
-  uint32_t block_size;     /* threads per workgroup: 64..256 */
+  uint32_t persistent;     /* reserved (waves are always persistent on an atomic ray queue) */
+  uint32_t block_size;     /* threads per workgroup: 64, 128 or 256 */
+  uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves) */
 } j3dg_tuning;
 
 /* Statistics of the last j3dg_render / j3dg_render_batch. */
@@ -120,7 +121,14 @@ typedef struct j3dg_stats {
   uint64_t learned_resolved;         /* primary rays resolved by the bounded pass */
   uint64_t learned_fallback;         /* primary rays that needed the plain pass */
   uint64_t stack_overflow;           /* must be 0 */
-  float trace_ms, shade_ms, mlp_ms;  /* HIP-event times of the last launch of each kernel */
+  uint64_t wave_trips, lane_steps;   /* traversal loop trips per wave and steps per lane (counting on) */
+  uint64_t shadow_node_visits, shadow_tri_tests, shadow_wave_trips, shadow_lane_steps;
+  uint64_t max_ray_steps, max_shadow_ray_steps;   /* longest ray, in traversal steps (counting on) */
+  uint64_t ray_steps_log2[7], shadow_ray_steps_log2[7]; /* rays with floor(log2(steps)) = 0..5, >= 6 */
+  float trace_ms;                    /* primary-ray trace kernel (HIP events, last render) */
+  float shade_ms;                    /* shading/edge kernel */
+  float mlp_ms;                      /* learned proposal inference */
+  float shadow_ms;                   /* k_finish + shadow-ray trace */
 } j3dg_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -135,6 +143,12 @@ int j3dg_get_tuning(j3dg_ctx* ctx, j3dg_tuning* t);
 /* Per-ray node/triangle counters in the trace kernel (roofline accounting). */
 int j3dg_set_counting(j3dg_ctx* ctx, int on);
 int j3dg_get_stats(j3dg_ctx* ctx, j3dg_stats* st);
+/* Per-render kernel durations (HIP events on the context's stream) of every
+   render since the last reset (up to 1024): primary trace, shade, MLP and
+   finish + shadow trace, in milliseconds (any pointer may be NULL). */
+int j3dg_reset_timings(j3dg_ctx* ctx);
+int j3dg_get_timings(j3dg_ctx* ctx, float* trace_ms, float* shade_ms, float* mlp_ms, float* shadow_ms, uint32_t max,
+                     uint32_t* n);
 
 /* ---- scene ------------------------------------------------------------ */
 /* Replaces scene.cpp:8-25 add_object for a mesh: computes triangle normals
@@ -181,6 +195,12 @@ int j3dg_trace_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, cons
                     uint32_t* obj, uint32_t* tri, float* t, float* u, float* v);
 int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
                       uint8_t* hit);
+
+/* ---- measurement ---------------------------------------------------------
+   A streaming device-to-device copy of `bytes` (read bytes, write bytes),
+   repeated `repeats` times: the reference point that scales profiler byte
+   counters to HBM bytes.  *ms receives the mean duration of one copy. */
+int j3dg_calibration_copy(j3dg_ctx* ctx, uint64_t bytes, uint32_t repeats, float* ms);
 
 #ifdef __cplusplus
 }

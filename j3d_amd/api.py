@@ -301,8 +301,8 @@ class Renderer:
     def synchronize(self):
         self._c(self.lib.j3dg_synchronize(self.ctx), "j3dg_synchronize")
 
-    def set_tuning(self, stack_depth=16, rays_per_item=1, persistent=1, block_size=256):
-        t = ffi.Tuning(stack_depth, rays_per_item, persistent, block_size)
+    def set_tuning(self, stack_depth=16, rays_per_item=1, persistent=1, block_size=256, refill_min=16):
+        t = ffi.Tuning(stack_depth, rays_per_item, persistent, block_size, refill_min)
         self._c(self.lib.j3dg_set_tuning(self.ctx, C.byref(t)), "j3dg_set_tuning")
 
     def get_tuning(self):
@@ -316,7 +316,32 @@ class Renderer:
     def stats(self):
         s = ffi.Stats()
         self._c(self.lib.j3dg_get_stats(self.ctx, C.byref(s)), "j3dg_get_stats")
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        out = {}
+        for k, _ in s._fields_:
+            v = getattr(s, k)
+            out[k] = list(v) if k.endswith("_log2") else v
+        return out
+
+    def reset_timings(self):
+        self._c(self.lib.j3dg_reset_timings(self.ctx), "j3dg_reset_timings")
+
+    def timings(self, max_n=1024):
+        """Per-render kernel durations (ms) since the last reset: dict of arrays."""
+        tr = np.zeros(max_n, np.float32)
+        sh = np.zeros(max_n, np.float32)
+        ml = np.zeros(max_n, np.float32)
+        sd = np.zeros(max_n, np.float32)
+        n = C.c_uint32(0)
+        self._c(self.lib.j3dg_get_timings(self.ctx, ffi.ptr(tr), ffi.ptr(sh), ffi.ptr(ml), ffi.ptr(sd), max_n,
+                                          C.byref(n)), "j3dg_get_timings")
+        k = n.value
+        return {"trace_ms": tr[:k], "shade_ms": sh[:k], "mlp_ms": ml[:k], "shadow_ms": sd[:k]}
+
+    def calibration_copy(self, nbytes, repeats=1):
+        ms = C.c_float(0)
+        self._c(self.lib.j3dg_calibration_copy(self.ctx, int(nbytes), int(repeats), C.byref(ms)),
+                "j3dg_calibration_copy")
+        return ms.value
 
     def trace_rays(self, slots, rays):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -203,19 +203,83 @@ inline void widen(const Aabb& b, float* lo, float* hi) {
   }
 }
 
+// Writes one quantized wide node (layout in bvh_build.h).  Child boxes are
+// given widened; quantization rounds them outward (floor / ceil), so the
+// decoded box always contains the widened box.
+void write_node(uint32_t* nd, int W, int n, const float (*lo)[3], const float (*hi)[3], const uint32_t* refs) {
+  const int words = node_words(W);
+  for (int i = 0; i < words; ++i) nd[i] = 0;
+  float p[3] = {0.f, 0.f, 0.f}, nh[3] = {0.f, 0.f, 0.f};
+  for (int k = 0; k < 3; ++k) {
+    p[k] = INFINITY;
+    nh[k] = -INFINITY;
+    for (int i = 0; i < n; ++i) {
+      p[k] = std::min(p[k], lo[i][k]);
+      nh[k] = std::max(nh[k], hi[i][k]);
+    }
+    if (n == 0) p[k] = nh[k] = 0.f;
+  }
+  uint32_t eb[3];
+  uint8_t q[2][3][8];  // [lo/hi][axis][child]
+  for (int k = 0; k < 3; ++k) {
+    const double ext = (double)nh[k] - (double)p[k];
+    int e = -126;
+    if (ext > 0.0) {
+      e = (int)std::ceil(std::log2(ext / 255.0));
+      if (e < -126) e = -126;
+      while ((double)p[k] + 255.0 * std::ldexp(1.0, e) < (double)nh[k]) ++e;
+    }
+    eb[k] = (uint32_t)(e + 127);
+    const double sc = std::ldexp(1.0, e);
+    for (int i = 0; i < 8; ++i) {
+      if (i >= n) {  // empty slot: lo > hi on every axis, never entered
+        q[0][k][i] = 255;
+        q[1][k][i] = 0;
+        continue;
+      }
+      double ql = std::floor(((double)lo[i][k] - (double)p[k]) / sc);
+      double qh = std::ceil(((double)hi[i][k] - (double)p[k]) / sc);
+      while (ql > 0 && (double)p[k] + ql * sc > (double)lo[i][k]) ql -= 1;
+      while ((double)p[k] + qh * sc < (double)hi[i][k]) qh += 1;
+      q[0][k][i] = (uint8_t)std::min(std::max(ql, 0.0), 255.0);
+      q[1][k][i] = (uint8_t)std::min(std::max(qh, 0.0), 255.0);
+    }
+  }
+  std::memcpy(&nd[0], &p[0], 4);
+  std::memcpy(&nd[1], &p[1], 4);
+  std::memcpy(&nd[2], &p[2], 4);
+  nd[3] = eb[0] | (eb[1] << 8) | (eb[2] << 16);
+  auto pack4 = [&](int side, int k, int first) {
+    return (uint32_t)q[side][k][first] | ((uint32_t)q[side][k][first + 1] << 8) |
+           ((uint32_t)q[side][k][first + 2] << 16) | ((uint32_t)q[side][k][first + 3] << 24);
+  };
+  if (W == 4) {
+    nd[4] = pack4(0, 0, 0), nd[5] = pack4(1, 0, 0);
+    nd[6] = pack4(0, 1, 0), nd[7] = pack4(1, 1, 0);
+    nd[8] = pack4(0, 2, 0), nd[9] = pack4(1, 2, 0);
+    for (int i = 0; i < 4; ++i) nd[10 + i] = i < n ? refs[i] : kEmptyRef;
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      nd[4 + 4 * k] = pack4(0, k, 0), nd[5 + 4 * k] = pack4(0, k, 4);
+      nd[6 + 4 * k] = pack4(1, k, 0), nd[7 + 4 * k] = pack4(1, k, 4);
+    }
+    for (int i = 0; i < 8; ++i) nd[16 + i] = i < n ? refs[i] : kEmptyRef;
+  }
+}
+
 struct Collapser {
   const BinNode* bn;
   int W;
-  std::vector<float>& out;
+  std::vector<uint32_t>& out;
   uint32_t n_wide = 0;
   uint32_t max_depth = 0, max_stack = 0;
 
-  Collapser(const BinNode* b, int w, std::vector<float>& o) : bn(b), W(w), out(o) {}
+  Collapser(const BinNode* b, int w, std::vector<uint32_t>& o) : bn(b), W(w), out(o) {}
 
   uint32_t alloc(uint32_t count) {
     uint32_t first = n_wide;
     n_wide += count;
-    out.resize((size_t)n_wide * 8 * W);
+    out.resize((size_t)n_wide * node_words(W));
     return first;
   }
 
@@ -256,38 +320,22 @@ struct Collapser {
     for (uint32_t c : ch)
       if (!bn[c].leaf) ++n_int;
     uint32_t first = n_int ? alloc(n_int) : 0;
-    float* nd = &out[(size_t)w * 8 * W];
-    uint32_t* refs = reinterpret_cast<uint32_t*>(nd + 6 * W);
-    for (int i = 0; i < W; ++i) {
-      nd[0 * W + i] = INFINITY;
-      nd[1 * W + i] = -INFINITY;
-      nd[2 * W + i] = INFINITY;
-      nd[3 * W + i] = -INFINITY;
-      nd[4 * W + i] = INFINITY;
-      nd[5 * W + i] = -INFINITY;
-      refs[i] = kEmptyRef;
-      nd[7 * W + i] = 0.f;
-    }
+    float lo[8][3], hi[8][3];
+    uint32_t refs[8];
     std::vector<std::pair<uint32_t, uint32_t>> todo;
     uint32_t k = 0;
     for (size_t i = 0; i < ch.size(); ++i) {
       const BinNode& c = bn[ch[i]];
-      float lo[3], hi[3];
-      widen(c.box, lo, hi);
-      nd[0 * W + i] = lo[0];
-      nd[1 * W + i] = hi[0];
-      nd[2 * W + i] = lo[1];
-      nd[3 * W + i] = hi[1];
-      nd[4 * W + i] = lo[2];
-      nd[5 * W + i] = hi[2];
+      widen(c.box, lo[i], hi[i]);
       if (c.leaf) {
-        refs[i] = kLeafBit | c.a;
+        refs[i] = kLeafBit | ((c.b - 1u) << kLeafCountShift) | c.a;
       } else {
         refs[i] = first + k;
         todo.emplace_back(first + k, ch[i]);
         ++k;
       }
     }
+    write_node(&out[(size_t)w * node_words(W)], W, (int)ch.size(), lo, hi, refs);
     for (auto& t : todo) fill(t.first, t.second, depth + 1, pushed);
   }
 };
@@ -301,8 +349,8 @@ bool build_bvh(const float* xyz, uint64_t n_vertices, const uint32_t* tris, uint
     return false;
   };
   if (opt.width != 4 && opt.width != 8) return fail("BVH width must be 4 or 8");
-  if (opt.leaf_size < 1 || opt.leaf_size > 16) return fail("leaf size must be in [1, 16]");
-  if (n_tris >= 0x7FFFFFFEull) return fail("too many triangles (max 2^31-2)");
+  if (opt.leaf_size < 1 || opt.leaf_size > 4) return fail("leaf size must be in [1, 4]");
+  if (n_tris > kLeafIndexMask) return fail("too many triangles (max 2^29-1 per mesh)");
   int threads = opt.threads > 0 ? opt.threads : (int)std::max(1u, std::thread::hardware_concurrency());
   for (uint64_t i = 0; i < 3 * n_tris; ++i)
     if (tris[i] >= n_vertices) return fail("triangle index out of range");
@@ -359,14 +407,7 @@ bool build_bvh(const float* xyz, uint64_t n_vertices, const uint32_t* tris, uint
   if (nt) {
     C.fill(0, 0, 1, 0);
   } else {
-    float* nd = &out.nodes[0];
-    uint32_t* refs = reinterpret_cast<uint32_t*>(nd + 6 * W);
-    for (int i = 0; i < W; ++i) {
-      nd[0 * W + i] = nd[2 * W + i] = nd[4 * W + i] = INFINITY;
-      nd[1 * W + i] = nd[3 * W + i] = nd[5 * W + i] = -INFINITY;
-      refs[i] = kEmptyRef;
-      nd[7 * W + i] = 0.f;
-    }
+    write_node(&out.nodes[0], W, 0, nullptr, nullptr, nullptr);
     C.max_depth = 1;
   }
   out.n_nodes = C.n_wide;

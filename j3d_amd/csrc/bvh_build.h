@@ -1,16 +1,21 @@
 // bvh_build.h -- host BVH builder and GPU re-pack (replaces the jtk::qbvh
 // build at j3d/scene.cpp:23 with this library's own builder).
 //
-// Device layout (all arrays 16-byte aligned, one node per cache line):
+// Device layout (16-byte aligned arrays):
 //
-//   wide node, W = 4 (128 B) or 8 (256 B), float words:
-//     [0W..1W) lo.x  [1W..2W) hi.x  [2W..3W) lo.y  [3W..4W) hi.y
-//     [4W..5W) lo.z  [5W..6W) hi.z  [6W..7W) child refs (uint32)  [7W..8W) unused
+//   quantized wide node, W = 4 (64 B, four 16-byte loads) or 8 (96 B, six):
+//     word 0..2  p = node origin (float x, y, z): the minimum of the children's boxes
+//     word 3     biased exponents e_x | e_y << 8 | e_z << 16: the quantum of axis k is 2^(e_k - 127)
+//     W = 4:  word 4 qlo.x[4]  5 qhi.x[4]  6 qlo.y  7 qhi.y  8 qlo.z  9 qhi.z   (one byte per child)
+//             word 10..13 child refs, 14..15 unused
+//     W = 8:  words 4..7 qlo.x[8], qhi.x[8]; 8..11 y; 12..15 z;  words 16..23 child refs
+//   child box on axis k = [p_k + qlo * 2^e, p_k + qhi * 2^e]; qlo is rounded
+//   down and qhi up from the child's box widened by 2^-20 of its magnitude,
+//   so the traversal's box culling is conservative.  Empty slots have
+//   qlo = 255, qhi = 0 (never entered) and ref 0xFFFFFFFF.
 //   child ref: internal child  -> node index (bit 31 clear)
-//              leaf            -> 0x80000000 | index of the leaf's first triangle record
-//              empty slot      -> 0xFFFFFFFF with box lo = +inf, hi = -inf (never entered)
-//   Child boxes are widened by 2^-20 of their coordinate magnitude so that the
-//   traversal's box culling is conservative.
+//              leaf            -> 0x80000000 | (count - 1) << 29 | index of the leaf's first
+//                                 triangle record (count 1..4, index < 2^29)
 //
 //   triangle record (48 B), in leaf order:
 //     float4(v0.xyz, id)   id = original triangle index (uint32 bits)
@@ -25,13 +30,16 @@ namespace j3d {
 
 struct BuildOptions {
   int width = 4;      // 4 or 8
-  int leaf_size = 4;  // max triangles per leaf
+  int leaf_size = 4;  // max triangles per leaf (1..4)
   int threads = 0;    // 0 = hardware_concurrency
 };
 
+// 32-bit words per node.
+constexpr int node_words(int width) { return width == 8 ? 24 : 16; }
+
 struct BuiltBVH {
   int width = 4;
-  std::vector<float> nodes;  // n_nodes * 8 * width floats
+  std::vector<uint32_t> nodes;  // n_nodes * node_words(width)
   uint32_t n_nodes = 0;
   std::vector<float> tris;   // n_tris * 12 floats
   uint32_t max_depth = 0;    // wide levels, root = 1
@@ -40,6 +48,8 @@ struct BuiltBVH {
 };
 
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafIndexMask = (1u << 29) - 1u;
+constexpr int kLeafCountShift = 29;
 constexpr uint32_t kEmptyRef = 0xFFFFFFFFu;
 
 // Builds and re-packs.  Returns false on invalid input (index out of range,

@@ -21,348 +21,298 @@
 #include "j3dg.h"
 #include "j3dg_internal.h"
 #include "rt_math.h"
+#include "traverse.h"
 
 using namespace j3d;
 
 // ============================================================ device side ===
 
-namespace {
-
-constexpr uint32_t kMiss = 0xFFFFFFFFu;
-constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr float kSlackUp = 1.0f + 1.0f / 65536.0f;    // t comparisons: 2^-16 relative slack
-constexpr float kSlackDown = 1.0f - 1.0f / 65536.0f;
-
-struct Hit {
-  float t, u, v;
-  uint32_t tri, obj;
-  bool found;
-};
-
-struct Ray {
-  float ox, oy, oz, dx, dy, dz;
-  float ix, iy, iz;  // safe reciprocal direction
-  float t_near, t_far;
-};
-
-__device__ inline float safe_rcp(float d) {
-  float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
-  return 1.f / a;
-}
-
-__device__ inline Ray make_object_ray(const float* inv_cs, f4 o, f4 d, float tn, float tf) {
-  f4 oo = mat_vec(inv_cs, o);
-  f4 dd = mat_vec(inv_cs, d);
-  Ray r;
-  r.ox = oo.x, r.oy = oo.y, r.oz = oo.z;
-  r.dx = dd.x, r.dy = dd.y, r.dz = dd.z;
-  r.ix = safe_rcp(dd.x), r.iy = safe_rcp(dd.y), r.iz = safe_rcp(dd.z);
-  r.t_near = tn;
-  r.t_far = tf;
-  return r;
-}
-
-// Per-lane traversal stack: the top `D` entries live in LDS (interleaved by
-// lane, conflict-free), older entries spill to a global scratch area laid out
-// [entry][lane] so that spills of one wave coalesce.
-template <int D>
-struct Stack {
-  uint2* lds;         // &lds[0][tid], stride `stride`
-  uint32_t stride;    // threads per block
-  uint2* spill;       // &spill[0][global lane]
-  uint64_t sstride;   // total lanes
-  uint32_t cap;       // spill entries available
-  uint32_t sp;
-  bool overflow;
-
-  __device__ void push(uint32_t ref, float dist) {
-    uint32_t slot = sp & (D - 1);
-    if (sp >= (uint32_t)D) {
-      uint32_t e = sp - D;
-      if (e >= cap) {  // bounded by the builder's max_stack; never expected
-        overflow = true;
-        return;
-      }
-      spill[(uint64_t)e * sstride] = lds[slot * stride];
-    }
-    lds[slot * stride] = make_uint2(ref, __float_as_uint(dist));
-    ++sp;
-  }
-  __device__ bool pop(uint32_t& ref, float& dist) {
-    if (sp == 0) return false;
-    --sp;
-    uint32_t slot = sp & (D - 1);
-    uint2 v = lds[slot * stride];
-    if (sp >= (uint32_t)D) lds[slot * stride] = spill[(uint64_t)(sp - D) * sstride];
-    ref = v.x;
-    dist = __uint_as_float(v.y);
-    return true;
-  }
-};
-
-struct Counters {
-  uint64_t nodes, tris;
-};
-
-// Visits one wide node: pushes the hit children far-to-near, returns the nearest
-// (or kNone).  Culling is conservative (see the slack constants).
-template <int W, int D, bool COUNT>
-__device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t node, const Ray& r, float thi,
-                                      Stack<D>& st, Counters& c) {
-  const float4* np = nodes + (size_t)node * (2 * W);
-  float b[7 * W];
-#pragma unroll
-  for (int i = 0; i < 7 * W / 4; ++i) {
-    float4 q = np[i];
-    b[4 * i] = q.x, b[4 * i + 1] = q.y, b[4 * i + 2] = q.z, b[4 * i + 3] = q.w;
-  }
-  if (COUNT) c.nodes++;
-  const float tlo = r.t_near * kSlackDown;
-  float dist[W];
-  uint32_t ref[W];
-  const bool sx = r.ix < 0.f, sy = r.iy < 0.f, sz = r.iz < 0.f;
-#pragma unroll
-  for (int i = 0; i < W; ++i) {
-    float nx = sx ? b[1 * W + i] : b[0 * W + i], fx = sx ? b[0 * W + i] : b[1 * W + i];
-    float ny = sy ? b[3 * W + i] : b[2 * W + i], fy = sy ? b[2 * W + i] : b[3 * W + i];
-    float nz = sz ? b[5 * W + i] : b[4 * W + i], fz = sz ? b[4 * W + i] : b[5 * W + i];
-    float tnx = (nx - r.ox) * r.ix, tfx = (fx - r.ox) * r.ix;
-    float tny = (ny - r.oy) * r.iy, tfy = (fy - r.oy) * r.iy;
-    float tnz = (nz - r.oz) * r.iz, tfz = (fz - r.oz) * r.iz;
-    float en = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tlo));
-    float ex = fminf(fminf(fminf(tfx, tfy), tfz) * kSlackUp, thi);
-    dist[i] = en <= ex ? en : INFINITY;
-    ref[i] = __float_as_uint(b[6 * W + i]);
-  }
-  // sorting network, ascending distance
-#define J3D_CSWAP(a, bb)                                      \
-  if (dist[bb] < dist[a]) {                                    \
-    float td = dist[a];                                        \
-    dist[a] = dist[bb];                                        \
-    dist[bb] = td;                                             \
-    uint32_t tr = ref[a];                                      \
-    ref[a] = ref[bb];                                          \
-    ref[bb] = tr;                                              \
-  }
-  if (W == 4) {
-    J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(1, 2)
-  } else {
-    J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(4, 6) J3D_CSWAP(5, 7)
-    J3D_CSWAP(0, 4) J3D_CSWAP(1, 5) J3D_CSWAP(2, 6) J3D_CSWAP(3, 7)
-    J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(4, 5) J3D_CSWAP(6, 7)
-    J3D_CSWAP(2, 4) J3D_CSWAP(3, 5)
-    J3D_CSWAP(1, 4) J3D_CSWAP(3, 6)
-    J3D_CSWAP(1, 2) J3D_CSWAP(3, 4) J3D_CSWAP(5, 6)
-  }
-#undef J3D_CSWAP
-#pragma unroll
-  for (int i = W - 1; i >= 1; --i)
-    if (dist[i] != INFINITY) st.push(ref[i], dist[i]);
-  return dist[0] != INFINITY ? ref[0] : kNone;
-}
-
-// Closest hit (or any hit) inside one object.  `best` carries the running
-// lexicographic minimum of (t, object index, triangle id) across objects.
-template <int W, int D, bool ANY, bool COUNT>
-__device__ bool trace_object(const float4* __restrict__ nodes, const float4* __restrict__ tris, uint32_t oi,
-                             const Ray& r, Hit& best, Stack<D>& st, Counters& c) {
-  float tmax = best.found ? fminf(best.t, r.t_far) : r.t_far;
-  float thi = tmax * kSlackUp;
-  uint32_t cur = 0;  // root is an internal node
-  st.sp = 0;
-  for (;;) {
-    if (cur == kNone) {
-      uint32_t ref;
-      float dist;
-      bool got = false;
-      while (st.pop(ref, dist)) {
-        if (dist <= thi) {
-          got = true;
-          break;
-        }
-      }
-      if (!got) break;
-      cur = ref;
-    }
-    if (cur & kLeafBit) {
-      uint32_t k = cur & ~kLeafBit;
-      for (;;) {
-        const float4* tp = tris + (size_t)k * 3;
-        float4 a = tp[0], e1 = tp[1], e2 = tp[2];
-        if (COUNT) c.tris++;
-        float t, u, v;
-        if (intersect_triangle(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, a.x, a.y, a.z, e1.x, e1.y, e1.z, e2.x, e2.y,
-                               e2.z, r.t_near, tmax, t, u, v)) {
-          if (ANY) return true;
-          uint32_t id = __float_as_uint(a.w);
-          bool better = !best.found || t < best.t || (t == best.t && (oi < best.obj || (oi == best.obj && id < best.tri)));
-          if (better) {
-            best.found = true;
-            best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = oi;
-            tmax = t;
-            thi = t * kSlackUp;
-          }
-        }
-        if (__float_as_uint(e1.w) & 1u) break;
-        ++k;
-      }
-      cur = kNone;
-    } else {
-      cur = visit_node<W, D, COUNT>(nodes, cur, r, thi, st, c);
-    }
-  }
-  return false;
-}
-
-}  // namespace
-
 namespace j3d {
 
-// Primary-ray tile of 8x8 pixels per wave.
-__device__ inline bool tile_pixel(const FrameParams& f, uint32_t tile, uint32_t lane, uint32_t& px, uint32_t& py) {
+// Primary rays come in tiles of 8x8 pixels: ray id = tile * 64 + lane.
+__device__ inline bool tile_pixel(const FrameParams& f, uint32_t tile, uint32_t l, uint32_t& px, uint32_t& py) {
   uint32_t tx = tile % f.tiles_x, ty = tile / f.tiles_x;
-  px = (uint32_t)f.x0 + tx * 8 + (lane & 7);
-  py = (uint32_t)f.y0 + ty * 8 + (lane >> 3);
+  px = (uint32_t)f.x0 + tx * 8 + (l & 7);
+  py = (uint32_t)f.y0 + ty * 8 + (l >> 3);
   return px <= (uint32_t)f.x1 && py <= (uint32_t)f.y1;
 }
 
-template <int W, int D, bool COUNT>
-__device__ void trace_pixel(const FrameParams& f, const DevObject* __restrict__ objs, uint32_t px, uint32_t py,
-                            Stack<D>& st, Counters& c, uint32_t* lstats) {
-  const float Wf = (float)f.w, Hf = (float)f.h;
-  // j3d/canvas.cpp:773-782
-  f4 sp = {2.f * (((float)px + 0.5f) / Wf) - 1.f, 2.f * (((float)py + 0.5f) / Hf) - 1.f, f.near_plane, 1.f};
-  f4 dir = mat_vec(f.proj_inv, sp);
-  dir.w = 0.f;
-  dir = mat_vec(f.cs, dir);
-  const f4 origin = mat_vec(f.cs, f4{0.f, 0.f, 0.f, 1.f});
-  const float t_near = f.diagonal / 100.f;
+enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3 };
+
+// Event bits a lane raises during one loop trip; the wave turns them into
+// uniform counts with ballot + popcount (no per-lane counters).
+enum : uint32_t { kEvPrimary = 1u, kEvHit = 2u, kEvShadowed = 4u, kEvResolved = 8u, kEvFallback = 16u };
+
+__device__ inline void add_counter(uint64_t* counters, int k, uint64_t v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane_id() == 0 && v) atomicAdd((unsigned long long*)&counters[k], (unsigned long long)v);
+}
+
+// Persistent waves with per-lane refill.  Counters (uint64):
+//   0 primary rays  1 hits  2 shadow rays  3 shadowed  4 node visits  5 triangle tests
+//   6 stack overflows  7 learned: resolved by the bounded pass  8 learned: fell back
+//   9 wave loop trips  10 lane steps
+//
+// kModeFrame: primary rays of FrameParams' rectangle.  A hit writes depth,
+//   object_id and the barycentrics and parks the object index in db_id for
+//   k_finish; a miss writes the final miss fields (j3d/canvas.cpp:859-866).
+// kModeShadow: any-hit rays from the shadow queue k_finish filled; a hit sets
+//   the pixel's mark bit 0 (j3d/canvas.cpp:855-856).
+// kModeRays / kModeAnyRays: world-space rays from a buffer (ray queries).
+template <int W, int D, bool COUNT, int MODE>
+__global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32_t n_rays_q,
+                                               const DevObject* __restrict__ objs, uint32_t n_obj, uint2* spill,
+                                               uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
+                                               uint32_t batch, uint32_t refill_min) {
+  extern __shared__ uint2 lds_stack[];
+  Stack<D> st;
+  st.init(lds_stack, spill, spill_cap);
+  Counters c = {0, 0, 0};
+  uint32_t acc[5] = {0, 0, 0, 0, 0};  // wave-uniform event counts
+  uint32_t trips = 0;
+  uint32_t ray_steps = 0;            // counting: steps of the current ray
+  RayPool pool;
+  pool.init();
+  uint32_t n_rays = n_rays_q;
+  if (MODE == kModeFrame) n_rays = f.n_tiles * 64u;
+  if (MODE == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
+  const bool learned = MODE == kModeFrame && (f.flags & J3DG_LEARNED) && f.proposal;
+  const bool any_mode = MODE == kModeAnyRays || MODE == kModeShadow;
+
+  // lane state
+  uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain
+  uint32_t pix = 0;    // pixel index (frames, shadow queue) or ray id (queries)
+  f4 wo = {0.f, 0.f, 0.f, 1.f}, wd = {0.f, 0.f, 0.f, 0.f};
+  Trav s;
+  s.any = any_mode;
+  s.cur = kNone;
+  s.obj = 0;
   Hit best;
   best.found = false;
-  best.t = 0.f, best.u = 0.f, best.v = 0.f, best.tri = kMiss, best.obj = kMiss;
-  for (uint32_t i = 0; i < f.n_obj; ++i) {
-    const DevObject& ob = objs[i];
-    Ray r = make_object_ray(ob.inv_cs, origin, dir, t_near, FLT_MAX);
-    trace_object<W, D, false, COUNT>(ob.nodes, ob.tris, i, r, best, st, c);
-  }
-  lstats[0]++;
-  char* pix = reinterpret_cast<char*>(f.gbuf) + ((size_t)py * f.w + px) * 32;
-  if (best.found) {
-    lstats[1]++;
-    const DevObject& ob = objs[best.obj];
-    // j3d/canvas.cpp:790-801
-    const float* tn = ob.normals + 3 * (size_t)best.tri;
-    f4 n = {tn[0], tn[1], tn[2], 0.f};
-    n = mat_vec(f.cs_inv, n);
-    n = mat_vec(ob.cs, n);
-    uint32_t mark = 0;
-    if (f.flags & J3DG_SHADOW) {
-      // j3d/canvas.cpp:836-857 (any hit decides, only hit2.found is used)
-      const uint32_t* tr = ob.idx + 3 * (size_t)best.tri;
-      f4 V[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float* vv = ob.verts + 3 * (size_t)tr[k];
-        V[k] = mat_vec(ob.cs, f4{vv[0], vv[1], vv[2], 1.f});
-      }
-      float s0 = 1.f - best.u - best.v;
-      f4 pos = {V[0].x * s0 + best.u * V[1].x + best.v * V[2].x, V[0].y * s0 + best.u * V[1].y + best.v * V[2].y,
-                V[0].z * s0 + best.u * V[1].z + best.v * V[2].z, V[0].w * s0 + best.u * V[1].w + best.v * V[2].w};
-      f4 ld;
-      if (f.flags & J3DG_DIRECTIONAL_LIGHT)
-        ld = f4{f.light_dir[0], f.light_dir[1], f.light_dir[2], 0.f};
-      else
-        ld = f4{f.light[0] - pos.x, f.light[1] - pos.y, f.light[2] - pos.z, f.light[3] - pos.w};
-      lstats[2]++;
-      Hit sh;
-      sh.found = false;
-      for (uint32_t i = 0; i < f.n_obj; ++i) {
-        const DevObject& o2 = objs[i];
-        Ray r = make_object_ray(o2.inv_cs, pos, ld, 1e-3f, FLT_MAX);
-        if (trace_object<W, D, true, COUNT>(o2.nodes, o2.tris, i, r, sh, st, c)) {
-          mark |= 1u;
-          break;
+  best.t = best.u = best.v = 0.f;
+  best.tri = best.obj = kMiss;
+
+  for (;;) {
+    uint32_t ev = 0;
+    uint32_t n_idle = (uint32_t)__popcll(__ballot(phase == 0));
+    if (n_idle == 64 && pool.exhausted) break;
+    if (!pool.exhausted && (n_idle >= refill_min || n_idle == 64)) {
+      // hand new rays to idle lanes; ids outside the rectangle leave a lane idle, so repeat
+      while (__ballot(phase == 0) && !pool.exhausted) {
+        uint32_t id = pool.take(phase == 0, queue, batch, n_rays);
+        if (id == kNone) continue;
+        float tn, tf;
+        if (MODE == kModeFrame) {
+          uint32_t px, py;
+          if (!tile_pixel(f, id >> 6, id & 63u, px, py)) continue;
+          // j3d/canvas.cpp:773-782
+          const float Wf = (float)f.w, Hf = (float)f.h;
+          f4 sp = {2.f * (((float)px + 0.5f) / Wf) - 1.f, 2.f * (((float)py + 0.5f) / Hf) - 1.f, f.near_plane, 1.f};
+          f4 dir = mat_vec(f.proj_inv, sp);
+          dir.w = 0.f;
+          wd = mat_vec(f.cs, dir);
+          wo = mat_vec(f.cs, f4{0.f, 0.f, 0.f, 1.f});
+          tn = f.diagonal / 100.f;
+          tf = FLT_MAX;
+          pix = py * f.w + px;
+          phase = 2;
+          if (learned) {
+            float2 pr = f.proposal[pix];
+            if (pr.y > 0.f) {  // predicted hit: bounded pass first
+              tf = pr.x * (1.f + f.margin);
+              phase = 1;
+            }
+          }
+          ev |= kEvPrimary;
+        } else if (MODE == kModeShadow) {
+          const float4* rr = reinterpret_cast<const float4*>(q.rays) + 3 * (size_t)id;
+          float4 o4 = gload4(rr), d4 = gload4(rr + 1), p4 = gload4(rr + 2);
+          wo = f4{o4.x, o4.y, o4.z, o4.w};
+          wd = f4{d4.x, d4.y, d4.z, d4.w};
+          tn = 1e-3f;
+          tf = FLT_MAX;
+          pix = __float_as_uint(p4.x);
+          phase = 2;
+        } else {
+          const float* rr = q.rays + 8 * (size_t)id;
+          wo = f4{rr[0], rr[1], rr[2], 1.f};
+          wd = f4{rr[3], rr[4], rr[5], 0.f};
+          tn = rr[6];
+          tf = rr[7];
+          pix = id;
+          phase = 2;
+        }
+        best.found = false;
+        st.sp = 0;
+        if (COUNT) ray_steps = 0;
+        if (n_obj) {
+          begin_object(s, objs, 0, wo, wd, tn, tf, best);
+        } else {
+          s.cur = kNone;
+          s.r.t_near = tn;
+          s.r.t_far = tf;
         }
       }
-      if (mark) lstats[3]++;
     }
-    *reinterpret_cast<uint8_t*>(pix) = (uint8_t)mark;
-    float* pf = reinterpret_cast<float*>(pix);
-    uint32_t* pu = reinterpret_cast<uint32_t*>(pix);
-    pf[1] = n.x;
-    pf[2] = n.y;
-    pf[3] = best.t;
-    pu[4] = best.tri;
-    pf[5] = best.u;
-    pf[6] = best.v;
-    pu[7] = ob.db_id;
-  } else {
-    // j3d/canvas.cpp:859-866: mark, r, g, b and barycentrics are left as they were
-    float* pf = reinterpret_cast<float*>(pix);
-    uint32_t* pu = reinterpret_cast<uint32_t*>(pix);
-    pf[1] = 0.f;
-    pf[2] = 0.f;
-    pf[3] = FLT_MAX;
-    pu[4] = kMiss;
-    pu[7] = 0u;
+    if (COUNT) trips++;
+    if (phase != 0) {
+      if (COUNT) ray_steps++;
+      bool done = n_obj == 0 || trav_step<W, D, COUNT>(s, best, st, c);
+      if (done && !(any_mode && best.found) && s.obj + 1 < n_obj) {
+        begin_object(s, objs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
+        st.sp = 0;
+        done = false;
+      }
+      if (done && COUNT && phase != 1) {
+        // per-ray step histogram: slot 16/17 = max steps (primary/shadow), 18.. = log2 buckets
+        const int hb = MODE == kModeShadow ? 25 : 18;
+        atomicMax((unsigned long long*)&counters[MODE == kModeShadow ? 17 : 16], (unsigned long long)ray_steps);
+        uint32_t b = 31u - __clz(ray_steps | 1u);
+        atomicAdd((unsigned long long*)&counters[hb + (b > 6 ? 6 : b)], 1ull);
+      }
+      if (done) {
+        if (MODE == kModeAnyRays) {
+          q.any[pix] = best.found ? 1 : 0;
+          phase = 0;
+        } else if (MODE == kModeRays) {
+          q.obj[pix] = best.found ? best.obj : kMiss;
+          q.tri[pix] = best.found ? best.tri : kMiss;
+          q.t[pix] = best.found ? best.t : FLT_MAX;
+          q.u[pix] = best.found ? best.u : 0.f;
+          q.v[pix] = best.found ? best.v : 0.f;
+          phase = 0;
+        } else if (MODE == kModeShadow) {
+          if (best.found) {
+            *(reinterpret_cast<uint8_t*>(f.gbuf) + (size_t)pix * 32) = 1;
+            ev |= kEvShadowed;
+          }
+          phase = 0;
+        } else if (phase == 1 && !best.found) {
+          // the bounded pass found nothing below t_pred * (1 + margin): exact fallback
+          ev |= kEvFallback;
+          phase = 2;
+          st.sp = 0;
+          begin_object(s, objs, 0, wo, wd, s.r.t_near, FLT_MAX, best);
+        } else {
+          if (phase == 1) ev |= kEvResolved;
+          phase = 0;
+          char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
+          float* pf = reinterpret_cast<float*>(pp);
+          uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
+          if (best.found) {
+            ev |= kEvHit;
+            pf[3] = best.t;
+            pu[4] = best.tri;
+            pf[5] = best.u;
+            pf[6] = best.v;
+            pu[7] = best.obj;  // object index; k_finish replaces it with the db_id
+          } else {
+            pf[1] = 0.f;
+            pf[2] = 0.f;
+            pf[3] = FLT_MAX;
+            pu[4] = kMiss;
+            pu[7] = 0u;
+          }
+        }
+      }
+    }
+    if ((MODE == kModeFrame || MODE == kModeShadow) && __ballot(ev != 0)) {
+      acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
+      acc[1] += (uint32_t)__popcll(__ballot(ev & kEvHit));
+      acc[2] += (uint32_t)__popcll(__ballot(ev & kEvShadowed));
+      acc[3] += (uint32_t)__popcll(__ballot(ev & kEvResolved));
+      acc[4] += (uint32_t)__popcll(__ballot(ev & kEvFallback));
+    }
   }
+  // node/triangle/trip/step counters: slots 4, 5, 9, 10 for primary and query
+  // launches, 12..15 for the shadow launch
+  const int cb = MODE == kModeShadow ? 12 : 4;
+  if (lane_id() == 0) {
+    const int slot[5] = {0, 1, 3, 7, 8};
+    for (int k = 0; k < 5; ++k)
+      if (acc[k]) atomicAdd((unsigned long long*)&counters[slot[k]], (unsigned long long)acc[k]);
+    if (COUNT) atomicAdd((unsigned long long*)&counters[MODE == kModeShadow ? 14 : 9], (unsigned long long)trips);
+  }
+  if (COUNT) {
+    add_counter(counters, cb, c.nodes);
+    add_counter(counters, cb + 1, c.tris);
+    add_counter(counters, MODE == kModeShadow ? 15 : 10, c.steps);
+  }
+  if (__any(st.overflow) && lane_id() == 0) atomicAdd((unsigned long long*)&counters[6], 1ull);
 }
 
-__device__ inline void wave_add_stats(uint64_t* counters, const Counters& c, const uint32_t* lstats, bool COUNT) {
-  // one atomic per wave and counter
-  uint64_t v[6] = {lstats[0], lstats[1], lstats[2], lstats[3], COUNT ? c.nodes : 0, COUNT ? c.tris : 0};
+// Completes the hit pixels of the rectangle (j3d/canvas.cpp:788-801, 836-853):
+// view-space normal, db_id and mark, and appends one shadow ray per hit pixel
+// to a compacted queue (wave ballot + one atomic per wave).
+__global__ void __launch_bounds__(256) k_finish(FrameParams f, const DevObject* __restrict__ objs,
+                                                float4* __restrict__ shadow_rays, uint32_t* shadow_count,
+                                                uint64_t* counters) {
+  const uint32_t rw = (uint32_t)(f.x1 - f.x0 + 1);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t rh = (uint32_t)(f.y1 - f.y0 + 1);
+  bool has_shadow = false;
+  f4 pos = {0.f, 0.f, 0.f, 0.f}, ld = {0.f, 0.f, 0.f, 0.f};
+  uint32_t pix = 0;
+  if (f.x1 >= f.x0 && f.y1 >= f.y0 && i < rw * rh) {
+    const uint32_t px = (uint32_t)f.x0 + i % rw, py = (uint32_t)f.y0 + i / rw;
+    pix = py * f.w + px;
+    char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
+    uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
+    float* pf = reinterpret_cast<float*>(pp);
+    const uint32_t tri = pu[4];
+    if (tri != kMiss) {
+      const uint32_t oi = pu[7];
+      const DevObject& ob = objs[oi];
+      const float* tnrm = ob.normals + 3 * (size_t)tri;
+      f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
+      n = mat_vec(f.cs_inv, n);
+      n = mat_vec(ob.cs, n);
+      *reinterpret_cast<uint8_t*>(pp) = 0;
+      pf[1] = n.x;
+      pf[2] = n.y;
+      pu[7] = ob.db_id;
+      if (f.flags & J3DG_SHADOW) {
+        const float bu = pf[5], bv = pf[6];
+        const uint32_t* tr = ob.idx + 3 * (size_t)tri;
+        f4 V[3];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    uint64_t s = v[k];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd((unsigned long long*)&counters[k], (unsigned long long)s);
-  }
-}
-
-// Grid-stride (static) or atomic-queue (dynamic) persistent waves over 8x8 tiles.
-template <int W, int D, bool COUNT, bool DYNAMIC>
-__global__ void __launch_bounds__(256) k_trace(FrameParams f, const DevObject* __restrict__ objs, uint2* spill,
-                                               uint32_t spill_cap, uint32_t* queue, uint64_t* counters) {
-  extern __shared__ uint2 lds_stack[];
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wave_in_block = threadIdx.x >> 6;
-  const uint32_t waves_per_block = blockDim.x >> 6;
-  const uint64_t total_lanes = (uint64_t)gridDim.x * blockDim.x;
-  Stack<D> st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = blockDim.x;
-  st.spill = spill + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
-  st.sstride = total_lanes;
-  st.cap = spill_cap;
-  st.sp = 0;
-  st.overflow = false;
-  Counters c = {0, 0};
-  uint32_t lstats[4] = {0, 0, 0, 0};
-  const uint32_t n_waves = gridDim.x * waves_per_block;
-  uint32_t tile = blockIdx.x * waves_per_block + wave_in_block;
-  if (DYNAMIC) {
-    uint32_t t0 = 0;
-    if (lane == 0) t0 = atomicAdd(queue, 1u);
-    tile = __shfl(t0, 0, 64);
-  }
-  while (tile < f.n_tiles) {
-    uint32_t px, py;
-    if (tile_pixel(f, tile, lane, px, py)) trace_pixel<W, D, COUNT>(f, objs, px, py, st, c, lstats);
-    if (DYNAMIC) {
-      uint32_t t0 = 0;
-      if (lane == 0) t0 = atomicAdd(queue, 1u);
-      tile = __shfl(t0, 0, 64);
-    } else {
-      tile += n_waves;
+        for (int k = 0; k < 3; ++k) {
+          const float* vv = ob.verts + 3 * (size_t)gload(tr + k);
+          V[k] = mat_vec(ob.cs, f4{gload(vv), gload(vv + 1), gload(vv + 2), 1.f});
+        }
+        float s0 = 1.f - bu - bv;
+        pos = f4{V[0].x * s0 + bu * V[1].x + bv * V[2].x, V[0].y * s0 + bu * V[1].y + bv * V[2].y,
+                 V[0].z * s0 + bu * V[1].z + bv * V[2].z, V[0].w * s0 + bu * V[1].w + bv * V[2].w};
+        if (f.flags & J3DG_DIRECTIONAL_LIGHT)
+          ld = f4{f.light_dir[0], f.light_dir[1], f.light_dir[2], 0.f};
+        else
+          ld = f4{f.light[0] - pos.x, f.light[1] - pos.y, f.light[2] - pos.z, f.light[3] - pos.w};
+        has_shadow = true;
+      }
     }
   }
-  wave_add_stats(counters, c, lstats, COUNT);
-  if (__any(st.overflow) && lane == 0) atomicAdd((unsigned long long*)&counters[6], 1ull);
+  uint64_t m = __ballot(has_shadow);
+  if (m) {
+    uint32_t base = 0;
+    if (lane_id() == 0) {
+      base = atomicAdd(shadow_count, (uint32_t)__popcll(m));
+      atomicAdd((unsigned long long*)&counters[2], (unsigned long long)__popcll(m));
+    }
+    base = __shfl(base, 0, 64);
+    if (has_shadow) {
+      float4* r = shadow_rays + 3 * (size_t)(base + rank_in_mask(m));
+      r[0] = make_float4(pos.x, pos.y, pos.z, pos.w);
+      r[1] = make_float4(ld.x, ld.y, ld.z, ld.w);
+      r[2] = make_float4(__uint_as_float(pix), 0.f, 0.f, 0.f);
+    }
+  }
 }
 
+// Clears the ray queues and counters of a launch (a kernel rather than a
+// memset so that it can live in a captured HIP graph).
 __global__ void k_reset(uint32_t* queue, uint64_t* counters, uint32_t n_counters) {
   uint32_t i = threadIdx.x;
-  if (i == 0) *queue = 0u;
+  if (i < 16) queue[i] = 0u;
   if (i < n_counters) counters[i] = 0ull;
 }
 
@@ -441,49 +391,11 @@ __global__ void __launch_bounds__(256) k_shade(ShadeParams s) {
   s.rgba[(size_t)y * s.w + x] = out;
 }
 
-// Raw ray queries (world-space rays, closest or any hit over a list of objects).
-template <int W, int D, bool ANY>
-__global__ void __launch_bounds__(256) k_trace_rays(const float* __restrict__ rays, uint64_t n,
-                                                    const DevObject* __restrict__ objs, uint32_t n_obj, uint2* spill,
-                                                    uint32_t spill_cap, uint32_t* out_obj, uint32_t* out_tri,
-                                                    float* out_t, float* out_u, float* out_v, uint8_t* out_any,
-                                                    uint64_t* counters) {
-  extern __shared__ uint2 lds_stack[];
-  const uint64_t total_lanes = (uint64_t)gridDim.x * blockDim.x;
-  Stack<D> st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = blockDim.x;
-  st.spill = spill + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
-  st.sstride = total_lanes;
-  st.cap = spill_cap;
-  st.sp = 0;
-  st.overflow = false;
-  Counters c = {0, 0};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += total_lanes) {
-    const float* rr = rays + 8 * i;
-    f4 o = {rr[0], rr[1], rr[2], 1.f}, d = {rr[3], rr[4], rr[5], 0.f};
-    Hit best;
-    best.found = false;
-    best.t = 0.f, best.u = 0.f, best.v = 0.f, best.tri = kMiss, best.obj = kMiss;
-    bool any = false;
-    for (uint32_t k = 0; k < n_obj; ++k) {
-      Ray r = make_object_ray(objs[k].inv_cs, o, d, rr[6], rr[7]);
-      if (trace_object<W, D, ANY, false>(objs[k].nodes, objs[k].tris, k, r, best, st, c)) {
-        any = true;
-        break;
-      }
-    }
-    if (ANY) {
-      out_any[i] = any ? 1 : 0;
-    } else {
-      out_obj[i] = best.found ? best.obj : kMiss;
-      out_tri[i] = best.found ? best.tri : kMiss;
-      out_t[i] = best.found ? best.t : FLT_MAX;
-      out_u[i] = best.found ? best.u : 0.f;
-      out_v[i] = best.found ? best.v : 0.f;
-    }
-  }
-  if (__any(st.overflow) && (threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&counters[6], 1ull);
+// Plain streaming copy of known size: the reference point for scaling the
+// profiler's HBM byte counters (FETCH_SIZE / WRITE_SIZE).
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_init_gbuf(j3dg_pixel* g, uint64_t n) {
@@ -506,6 +418,7 @@ __global__ void k_no_scene(j3dg_pixel* g, uint64_t n) {
   g[i].object_id = kMiss;
 }
 
+
 // ============================================================= host side ===
 
 namespace {
@@ -516,24 +429,28 @@ namespace {
     if (e_ != hipSuccess) return ctx->fail(code, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-template <int W, int D>
-struct TraceKernels {
-  static void* get(bool count, bool dynamic) {
-    if (count)
-      return dynamic ? (void*)&k_trace<W, D, true, true> : (void*)&k_trace<W, D, true, false>;
-    return dynamic ? (void*)&k_trace<W, D, false, true> : (void*)&k_trace<W, D, false, false>;
-  }
-};
+template <int W, int D, int MODE>
+void* trace_kernel_wd(bool count) {
+  return count ? (void*)&k_trace<W, D, true, MODE> : (void*)&k_trace<W, D, false, MODE>;
+}
 
-void* trace_kernel(int width, int depth, bool count, bool dynamic) {
+template <int MODE>
+void* trace_kernel_mode(int width, int depth, bool count) {
   if (width == 8) {
-    if (depth == 8) return TraceKernels<8, 8>::get(count, dynamic);
-    if (depth == 32) return TraceKernels<8, 32>::get(count, dynamic);
-    return TraceKernels<8, 16>::get(count, dynamic);
+    if (depth == 8) return trace_kernel_wd<8, 8, MODE>(count);
+    if (depth == 32) return trace_kernel_wd<8, 32, MODE>(count);
+    return trace_kernel_wd<8, 16, MODE>(count);
   }
-  if (depth == 8) return TraceKernels<4, 8>::get(count, dynamic);
-  if (depth == 32) return TraceKernels<4, 32>::get(count, dynamic);
-  return TraceKernels<4, 16>::get(count, dynamic);
+  if (depth == 8) return trace_kernel_wd<4, 8, MODE>(count);
+  if (depth == 32) return trace_kernel_wd<4, 32, MODE>(count);
+  return trace_kernel_wd<4, 16, MODE>(count);
+}
+
+void* trace_kernel(int mode, int width, int depth, bool count) {
+  if (mode == kModeRays) return trace_kernel_mode<kModeRays>(width, depth, false);
+  if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, false);
+  if (mode == kModeShadow) return trace_kernel_mode<kModeShadow>(width, depth, count);
+  return trace_kernel_mode<kModeFrame>(width, depth, count);
 }
 
 }  // namespace
@@ -599,40 +516,100 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
   return J3DG_OK;
 }
 
-int Ctx::launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned) {
-  (void)learned;
+uint32_t Ctx::trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const {
+  const uint32_t waves_per_block = block / 64;
+  const uint32_t lds = D * block * sizeof(uint2);
+  const uint32_t rays_per_block = 64u * waves_per_block;
+  uint32_t blocks_needed = (n_rays + rays_per_block - 1) / rays_per_block;
+  // persistent: as many resident blocks as LDS and the 32 waves per CU allow
+  uint32_t per_cu = std::max(1u, std::min(160u * 1024u / std::max(lds, 1u), 32u / waves_per_block));
+  return std::max(1u, std::min(blocks_needed, (uint32_t)n_cus * per_cu));
+}
+
+int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
+                             uint32_t max_stack, int width, uint32_t* qcounter) {
   Ctx* ctx = this;
   const uint32_t D = tuning.stack_depth;
   const uint32_t block = tuning.block_size;
-  const uint32_t waves_per_block = block / 64;
   const uint32_t lds = D * block * sizeof(uint2);
-  uint32_t blocks_needed = (fp.n_tiles + waves_per_block - 1) / waves_per_block;
-  uint32_t per_cu = std::max(1u, std::min(160u * 1024u / std::max(lds, 1u), 32u / waves_per_block));
-  uint32_t grid = std::max(1u, std::min(blocks_needed, (uint32_t)n_cus * per_cu));
+  uint32_t grid = trace_grid(n_rays, D, block);
   uint64_t lanes = (uint64_t)grid * block;
   uint32_t cap = max_stack > D ? max_stack - D + 1 : 1;
   int rc = ensure_spill(lanes, cap);
   if (rc) return rc;
-  void* fn = trace_kernel(width, (int)D, counting, tuning.persistent != 0);
+  void* fn = trace_kernel(mode, width, (int)D, counting);
   uint2* sp = spill;
-  uint32_t* q = d_queue;
+  uint32_t* qu = qcounter;
   uint64_t* cnt = d_counters;
   FrameParams f = fp;
+  RayQuery rq = q;
+  uint32_t nr = n_rays, no = n_obj;
+  uint32_t batch = 64u * tuning.rays_per_item;
+  uint32_t refill = tuning.refill_min;
   const DevObject* objs = d_objs;
-  void* args[] = {&f, &objs, &sp, &cap, &q, &cnt};
-  HIP_TRY(hipEventRecord(ev[0], stream), J3DG_ERR_LAUNCH);
+  void* args[] = {&f, &rq, &nr, &objs, &no, &sp, &cap, &qu, &cnt, &batch, &refill};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream), J3DG_ERR_LAUNCH);
-  HIP_TRY(hipEventRecord(ev[1], stream), J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
+int Ctx::ensure_shadow_queue(uint64_t rays) {
+  if (rays <= shadow_capacity) return J3DG_OK;
+  Ctx* ctx = this;
+  HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
+  if (d_shadow) hipFree(d_shadow);
+  d_shadow = nullptr;
+  shadow_capacity = 0;
+  HIP_TRY(hipMalloc(&d_shadow, rays * 3 * sizeof(float4)), J3DG_ERR_OUT_OF_MEMORY);
+  shadow_capacity = rays;
+  return J3DG_OK;
+}
+
+// Primary trace, k_finish (normals, db_id, shadow queue) and the shadow trace.
+int Ctx::launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned) {
+  (void)learned;
+  Ctx* ctx = this;
+  RayQuery q;
+  std::memset(&q, 0, sizeof(q));
+  hipEvent_t* e = cur_ev();
+  const uint32_t rect_px = fp.n_tiles * 64u;
+  HIP_TRY(hipEventRecord(e[0], stream), J3DG_ERR_LAUNCH);
+  int rc = launch_trace_kernel(kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width, d_queue);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(e[1], stream), J3DG_ERR_LAUNCH);
+  HIP_TRY(hipEventRecord(e[6], stream), J3DG_ERR_LAUNCH);
+  if (fp.n_obj) {
+    const bool shadow = (fp.flags & J3DG_SHADOW) != 0;
+    if (shadow) {
+      rc = ensure_shadow_queue(rect_px);
+      if (rc) return rc;
+    }
+    uint32_t rw = fp.x1 >= fp.x0 ? (uint32_t)(fp.x1 - fp.x0 + 1) : 0;
+    uint32_t rh = fp.y1 >= fp.y0 ? (uint32_t)(fp.y1 - fp.y0 + 1) : 0;
+    uint32_t n = rw * rh;
+    if (n) {
+      hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, stream, fp, (const DevObject*)d_objs,
+                         d_shadow, d_queue + 1, d_counters);
+      HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+    }
+    if (shadow && n) {
+      q.rays = reinterpret_cast<const float*>(d_shadow);
+      q.count = d_queue + 1;
+      rc = launch_trace_kernel(kModeShadow, fp, q, n, fp.n_obj, max_stack, width, d_queue + 2);
+      if (rc) return rc;
+    }
+  }
+  HIP_TRY(hipEventRecord(e[7], stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;
 }
 
 int Ctx::launch_shade(const ShadeParams& sp) {
   Ctx* ctx = this;
   dim3 grid((sp.w + 63) / 64, (sp.h + 3) / 4);
-  HIP_TRY(hipEventRecord(ev[2], stream), J3DG_ERR_LAUNCH);
+  hipEvent_t* e = cur_ev();
+  HIP_TRY(hipEventRecord(e[2], stream), J3DG_ERR_LAUNCH);
   hipLaunchKernelGGL(k_shade, grid, dim3(256), 0, stream, sp);
   HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
-  HIP_TRY(hipEventRecord(ev[3], stream), J3DG_ERR_LAUNCH);
+  HIP_TRY(hipEventRecord(e[3], stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;
 }
 
@@ -685,8 +662,11 @@ int j3dg_create(int device, j3dg_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
+  ctx->ring.assign(8 * Ctx::kRing, nullptr);
+  for (auto& e : ctx->ring)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
   if (hipMalloc(&ctx->d_queue, 64) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
-  if (hipMalloc(&ctx->d_counters, 16 * sizeof(uint64_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->d_counters, 32 * sizeof(uint64_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
   *out = reinterpret_cast<j3dg_ctx*>(ctx);
   return J3DG_OK;
 }
@@ -700,6 +680,8 @@ void j3dg_destroy(j3dg_ctx* h) {
   for (auto& t : ctx->targets) t.release();
   ctx->release_buffers();
   for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  for (auto& e : ctx->ring)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -725,6 +707,8 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
   if (n.stack_depth == 0) n.stack_depth = 16;
   if (n.rays_per_item == 0) n.rays_per_item = 1;
   if (n.block_size == 0) n.block_size = 256;
+  if (n.refill_min == 0) n.refill_min = 16;
+  if (n.refill_min > 64) return ctx->fail(J3DG_ERR_INVALID_ARG, "refill_min must be in [1, 64]");
   if (n.stack_depth != 8 && n.stack_depth != 16 && n.stack_depth != 32)
     return ctx->fail(J3DG_ERR_INVALID_ARG, "stack_depth must be 8, 16 or 32");
   if (n.rays_per_item != 1 && n.rays_per_item != 2 && n.rays_per_item != 4)
@@ -755,9 +739,19 @@ int j3dg_get_stats(j3dg_ctx* h, j3dg_stats* st) {
   CHECK_CTX();
   if (!st) return ctx->fail(J3DG_ERR_INVALID_ARG, "null stats");
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
-  uint64_t c[16];
+  uint64_t c[32];
   HIP_CALL(hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
   std::memset(st, 0, sizeof(*st));
+  st->shadow_node_visits = c[12];
+  st->shadow_tri_tests = c[13];
+  st->shadow_wave_trips = c[14];
+  st->shadow_lane_steps = c[15];
+  st->max_ray_steps = c[16];
+  st->max_shadow_ray_steps = c[17];
+  for (int k = 0; k < 7; ++k) {
+    st->ray_steps_log2[k] = c[18 + k];
+    st->shadow_ray_steps_log2[k] = c[25 + k];
+  }
   st->primary_rays = c[0];
   st->hits = c[1];
   st->shadow_rays = c[2];
@@ -767,10 +761,44 @@ int j3dg_get_stats(j3dg_ctx* h, j3dg_stats* st) {
   st->stack_overflow = c[6];
   st->learned_resolved = c[7];
   st->learned_fallback = c[8];
+  st->wave_trips = c[9];
+  st->lane_steps = c[10];
   float ms = 0.f;
-  if (ctx->traced && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) st->trace_ms = ms;
-  if (ctx->shaded && hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) st->shade_ms = ms;
-  if (ctx->mlp_ran && hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) st->mlp_ms = ms;
+  const hipEvent_t* e = ctx->last_ev ? ctx->last_ev : ctx->ev;
+  if (ctx->traced && hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) st->trace_ms = ms;
+  if (ctx->shaded && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) st->shade_ms = ms;
+  if (ctx->mlp_ran && hipEventElapsedTime(&ms, e[4], e[5]) == hipSuccess) st->mlp_ms = ms;
+  if (ctx->traced && hipEventElapsedTime(&ms, e[6], e[7]) == hipSuccess) st->shadow_ms = ms;
+  return J3DG_OK;
+}
+
+int j3dg_reset_timings(j3dg_ctx* h) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  ctx->ring_n = 0;
+  return J3DG_OK;
+}
+
+int j3dg_get_timings(j3dg_ctx* h, float* trace_ms, float* shade_ms, float* mlp_ms, float* shadow_ms, uint32_t max,
+                     uint32_t* n) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (!n) return ctx->fail(J3DG_ERR_INVALID_ARG, "null count");
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  uint32_t cnt = std::min(ctx->ring_n, Ctx::kRing);
+  uint32_t first = ctx->ring_n - cnt;
+  uint32_t k = 0;
+  for (uint32_t i = first; i < ctx->ring_n && k < max; ++i, ++k) {
+    const hipEvent_t* e = &ctx->ring[8 * (i % Ctx::kRing)];
+    uint32_t fl = ctx->ring_flags[i % Ctx::kRing];
+    float ms = 0.f;
+    if (trace_ms) trace_ms[k] = ((fl & 1) && hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) ? ms : 0.f;
+    if (shade_ms) shade_ms[k] = ((fl & 2) && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) ? ms : 0.f;
+    if (mlp_ms) mlp_ms[k] = ((fl & 4) && hipEventElapsedTime(&ms, e[4], e[5]) == hipSuccess) ? ms : 0.f;
+    if (shadow_ms) shadow_ms[k] = ((fl & 1) && hipEventElapsedTime(&ms, e[6], e[7]) == hipSuccess) ? ms : 0.f;
+  }
+  *n = k;
   return J3DG_OK;
 }
 
@@ -834,7 +862,7 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   auto t1 = std::chrono::steady_clock::now();
   m.info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
   m.normals_host = std::move(normals);
-  size_t nb = bvh.nodes.size() * sizeof(float), tb = bvh.tris.size() * sizeof(float);
+  size_t nb = bvh.nodes.size() * sizeof(uint32_t), tb = bvh.tris.size() * sizeof(float);
   size_t vb = 3 * n_vertices * sizeof(float), ib = 3 * n_triangles * sizeof(uint32_t);
   size_t mb = 3 * n_triangles * sizeof(float);
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
@@ -1018,7 +1046,7 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   ctx->traced = ctx->shaded = ctx->mlp_ran = false;
   const uint64_t npx = (uint64_t)tg.w * tg.h;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->d_queue, ctx->d_counters, 16u);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->d_queue, ctx->d_counters, 32u);
   HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
   if (stages & J3DG_STAGE_TRACE) {
     if (fd->flags & J3DG_NO_SCENE) {
@@ -1050,6 +1078,12 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
     if (rc) return rc;
     ctx->shaded = true;
   }
+  {
+    uint32_t fl = (ctx->traced ? 1u : 0u) | (ctx->shaded ? 2u : 0u) | (ctx->mlp_ran ? 4u : 0u);
+    ctx->last_ev = ctx->cur_ev();
+    ctx->ring_flags[ctx->ring_n % Ctx::kRing] = fl;
+    ctx->ring_n++;
+  }
   if (gbuf_out) HIP_CALL(hipMemcpyAsync(gbuf_out, tg.gbuf, npx * sizeof(j3dg_pixel), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
   if (rgba_out) HIP_CALL(hipMemcpyAsync(rgba_out, tg.rgba, npx * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
   if (gbuf_out || rgba_out) HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
@@ -1068,7 +1102,7 @@ static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
     int rc = ctx->build_object_table(slots, n_slots, max_stack, width);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->d_queue, ctx->d_counters, 16u);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->d_queue, ctx->d_counters, 32u);
   float* d_rays = nullptr;
   uint8_t* d_out = nullptr;
   size_t out_bytes = any ? n : n * 20;
@@ -1086,36 +1120,26 @@ static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
     }
   } guard{d_rays, d_out};
   HIP_CALL(hipMemcpyAsync(d_rays, rays, n * 8 * sizeof(float), hipMemcpyHostToDevice, ctx->stream), J3DG_ERR_LAUNCH);
-  const uint32_t D = 16, block = 256;
-  uint64_t blocks = (n + block - 1) / block;
-  uint32_t grid = (uint32_t)std::min<uint64_t>(blocks, (uint64_t)ctx->n_cus * 4);
-  uint32_t cap = max_stack > D ? max_stack - D + 1 : 1;
-  int rc = ctx->ensure_spill((uint64_t)grid * block, cap);
-  if (rc) return rc;
+  if (n > 0xFFFFFFFFull / 2) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many rays in one call");
   uint32_t* o_obj = reinterpret_cast<uint32_t*>(d_out);
   uint32_t* o_tri = o_obj + n;
   float* o_t = reinterpret_cast<float*>(o_tri + n);
   float* o_u = o_t + n;
   float* o_v = o_u + n;
-  uint8_t* o_any = d_out;
-  const DevObject* objs = ctx->d_objs;
-  uint32_t no = n_slots;
-  size_t lds = D * block * sizeof(uint2);
-  if (width == 8) {
-    if (any)
-      hipLaunchKernelGGL((k_trace_rays<8, 16, true>), dim3(grid), dim3(block), lds, ctx->stream, d_rays, n, objs, no, ctx->spill, cap, o_obj, o_tri, o_t, o_u, o_v, o_any, ctx->d_counters);
-    else
-      hipLaunchKernelGGL((k_trace_rays<8, 16, false>), dim3(grid), dim3(block), lds, ctx->stream, d_rays, n, objs, no, ctx->spill, cap, o_obj, o_tri, o_t, o_u, o_v, o_any, ctx->d_counters);
-  } else {
-    if (any)
-      hipLaunchKernelGGL((k_trace_rays<4, 16, true>), dim3(grid), dim3(block), lds, ctx->stream, d_rays, n, objs, no, ctx->spill, cap, o_obj, o_tri, o_t, o_u, o_v, o_any, ctx->d_counters);
-    else
-      hipLaunchKernelGGL((k_trace_rays<4, 16, false>), dim3(grid), dim3(block), lds, ctx->stream, d_rays, n, objs, no, ctx->spill, cap, o_obj, o_tri, o_t, o_u, o_v, o_any, ctx->d_counters);
-  }
+  RayQuery q;
+  q.rays = d_rays;
+  q.obj = o_obj, q.tri = o_tri, q.t = o_t, q.u = o_u, q.v = o_v;
+  q.any = d_out;
+  FrameParams fp;
+  std::memset(&fp, 0, sizeof(fp));
+  for (int i = 0; i < 16; ++i) fp.cs[i] = (i % 5 == 0) ? 1.f : 0.f;
+  int rc = ctx->launch_trace_kernel(any ? kModeAnyRays : kModeRays, fp, q, (uint32_t)n, n_slots, max_stack, width,
+                                    ctx->d_queue);
+  if (rc) return rc;
   HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
   if (any) {
-    HIP_CALL(hipMemcpy(hit, o_any, n, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
+    HIP_CALL(hipMemcpy(hit, d_out, n, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
   } else {
     if (obj) HIP_CALL(hipMemcpy(obj, o_obj, n * 4, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
     if (tri) HIP_CALL(hipMemcpy(tri, o_tri, n * 4, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
@@ -1123,7 +1147,7 @@ static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
     if (u) HIP_CALL(hipMemcpy(u, o_u, n * 4, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
     if (v) HIP_CALL(hipMemcpy(v, o_v, n * 4, hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
   }
-  uint64_t c[16];
+  uint64_t c[32];
   HIP_CALL(hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
   if (c[6]) return ctx->fail(J3DG_ERR_INTERNAL, "traversal stack overflow");
   return J3DG_OK;
@@ -1142,6 +1166,35 @@ int j3dg_any_hit_rays(j3dg_ctx* h, const uint32_t* slots, uint32_t n_slots, cons
   CHECK_CTX();
   if (!hit && n) return ctx->fail(J3DG_ERR_INVALID_ARG, "null output");
   return trace_rays_impl(ctx, slots, n_slots, rays, n, nullptr, nullptr, nullptr, nullptr, nullptr, hit, true);
+}
+
+int j3dg_calibration_copy(j3dg_ctx* h, uint64_t bytes, uint32_t repeats, float* ms) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (bytes < 16 || bytes % 16) return ctx->fail(J3DG_ERR_INVALID_ARG, "bytes must be a positive multiple of 16");
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  float4 *a = nullptr, *b = nullptr;
+  HIP_CALL(hipMalloc(&a, bytes), J3DG_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "calibration buffer");
+  }
+  uint64_t n = bytes / 16;
+  hipMemsetAsync(a, 1, bytes, ctx->stream);
+  hipEvent_t e0 = ctx->ev[0], e1 = ctx->ev[1];
+  hipEventRecord(e0, ctx->stream);
+  for (uint32_t r = 0; r < std::max(repeats, 1u); ++r)
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->n_cus * 16)), dim3(256),
+                       0, ctx->stream, a, b, n);
+  hipEventRecord(e1, ctx->stream);
+  hipError_t err = hipStreamSynchronize(ctx->stream);
+  float t = 0.f;
+  hipEventElapsedTime(&t, e0, e1);
+  if (ms) *ms = t / std::max(repeats, 1u);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) return ctx->fail(J3DG_ERR_LAUNCH, hipGetErrorString(err));
+  return J3DG_OK;
 }
 
 int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n) {

@@ -45,6 +45,18 @@ struct FrameParams {
   float margin;
 };
 
+// Ray-buffer queries (j3dg_trace_rays / j3dg_any_hit_rays).
+struct RayQuery {
+  const float* rays;  // n x 8
+  uint32_t* obj;
+  uint32_t* tri;
+  float* t;
+  float* u;
+  float* v;
+  uint8_t* any;
+  const uint32_t* count;  // shadow queue: number of rays (read on the device)
+};
+
 struct ShadeParams {
   uint32_t w, h;
   const j3dg_pixel* gbuf;
@@ -114,10 +126,18 @@ struct Ctx {
   std::string arch;
   std::string last_error;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // per-render HIP events (primary trace, shade, mlp, finish + shadow trace: start/end),
+  // kept for every render since the last j3dg_reset_timings
+  static constexpr uint32_t kRing = 1024;
+  std::vector<hipEvent_t> ring;    // kRing * 8
+  uint32_t ring_n = 0;             // renders recorded since reset
+  uint32_t ring_flags[kRing];      // bit 0 trace, bit 1 shade, bit 2 mlp
+  const hipEvent_t* last_ev = nullptr;
+  hipEvent_t* cur_ev() { return ring.empty() ? ev : &ring[8 * (ring_n % kRing)]; }
   bool traced = false, shaded = false, mlp_ran = false;
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 1, 256};
+  j3dg_tuning tuning{16, 1, 1, 256, 16};
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   uint32_t* d_queue = nullptr;
@@ -135,6 +155,12 @@ struct Ctx {
   int ensure_spill(uint64_t lanes, uint32_t entries);
   int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
   int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned);
+  int launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
+                          uint32_t max_stack, int width, uint32_t* qcounter);
+  int ensure_shadow_queue(uint64_t rays);
+  float4* d_shadow = nullptr;
+  uint64_t shadow_capacity = 0;
+  uint32_t trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const;
   int launch_shade(const ShadeParams& sp);
   void invalidate_graphs() {}
   void release_buffers() {
@@ -143,6 +169,8 @@ struct Ctx {
     if (spill) hipFree(spill);
     if (d_objs) hipFree(d_objs);
     if (d_matcap) hipFree(d_matcap);
+    if (d_shadow) hipFree(d_shadow);
+    d_shadow = nullptr;
     d_queue = nullptr, d_counters = nullptr, spill = nullptr, d_objs = nullptr, d_matcap = nullptr;
   }
 };

@@ -1,0 +1,342 @@
+// traverse.h -- device-side BVH traversal for gfx950.
+//
+// One ray per lane of a wave64.  The traversal is a per-lane state machine
+// advanced one step (one wide node or one leaf) per loop trip, so that a
+// persistent wave can hand a finished lane a new ray at once ("refill")
+// instead of idling it until the slowest lane of the wave is done.
+//
+// Closest hit is the lexicographic minimum of (t, object index, triangle id)
+// over every triangle the Moller-Trumbore test accepts, so the result does
+// not depend on the BVH topology or the visiting order.  Box culling is
+// conservative: child boxes are widened by 2^-20 of their magnitude at build
+// time and every t comparison carries a 2^-16 relative slack.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bvh_build.h"
+#include "j3dg_internal.h"
+#include "rt_math.h"
+
+namespace j3d {
+
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr float kSlackUp = 1.0f + 1.0f / 65536.0f;
+constexpr float kSlackDown = 1.0f - 1.0f / 65536.0f;
+
+struct Hit {
+  float t, u, v;
+  uint32_t tri, obj;
+  bool found;
+};
+
+struct Ray {
+  float ox, oy, oz, dx, dy, dz;
+  float ix, iy, iz;  // reciprocal direction, zero components replaced by +-1e-30
+  float t_near, t_far;
+};
+
+// Loads through the global address space.  Pointers read from the object table
+// are generic to the compiler, which would otherwise emit flat loads (they
+// wait on the LDS counter too, serialising them against the LDS stack).
+template <class T>
+__device__ inline T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+// float4 is a class type whose copy goes through a generic reference, so the
+// 16-byte loads use the native vector type.
+typedef float j3d_f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline float4 gload4(const float4* p) {
+  j3d_f32x4 v = *(const __attribute__((address_space(1))) j3d_f32x4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ inline float safe_rcp(float d) {
+  float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
+  return 1.f / a;
+}
+
+__device__ inline Ray make_object_ray(const float* inv_cs, f4 o, f4 d, float tn, float tf) {
+  f4 oo = mat_vec(inv_cs, o);
+  f4 dd = mat_vec(inv_cs, d);
+  Ray r;
+  r.ox = oo.x, r.oy = oo.y, r.oz = oo.z;
+  r.dx = dd.x, r.dy = dd.y, r.dz = dd.z;
+  r.ix = safe_rcp(dd.x), r.iy = safe_rcp(dd.y), r.iz = safe_rcp(dd.z);
+  r.t_near = tn;
+  r.t_far = tf;
+  return r;
+}
+
+// Per-lane traversal stack: the top D entries live in LDS (interleaved by lane,
+// so any mix of stack pointers is bank-conflict free); older entries spill to a
+// global scratch area laid out [entry][lane] so that one wave's spills coalesce.
+template <int D>
+struct Stack {
+  uint2* lds;        // &lds[0][tid], stride `stride`
+  uint32_t stride;   // threads per block
+  uint2* spill;      // &spill[0][global lane]
+  uint64_t sstride;  // lanes of the grid
+  uint32_t cap;      // spill entries per lane
+  uint32_t sp;
+  bool overflow;
+
+  __device__ void init(uint2* lds_base, uint2* spill_base, uint32_t spill_cap) {
+    lds = lds_base + threadIdx.x;
+    stride = blockDim.x;
+    sstride = (uint64_t)gridDim.x * blockDim.x;
+    spill = spill_base + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    cap = spill_cap;
+    sp = 0;
+    overflow = false;
+  }
+  __device__ void push(uint32_t ref, float dist) {
+    uint32_t slot = sp & (D - 1);
+    if (sp >= (uint32_t)D) {
+      uint32_t e = sp - D;
+      if (e >= cap) {  // cap comes from the builder's stack bound; never expected
+        overflow = true;
+        return;
+      }
+      spill[(uint64_t)e * sstride] = lds[slot * stride];
+    }
+    lds[slot * stride] = make_uint2(ref, __float_as_uint(dist));
+    ++sp;
+  }
+  __device__ bool pop(uint32_t& ref, float& dist) {
+    if (sp == 0) return false;
+    --sp;
+    uint32_t slot = sp & (D - 1);
+    uint2 v = lds[slot * stride];
+    if (sp >= (uint32_t)D) lds[slot * stride] = spill[(uint64_t)(sp - D) * sstride];
+    ref = v.x;
+    dist = __uint_as_float(v.y);
+    return true;
+  }
+};
+
+struct Counters {
+  uint32_t nodes, tris, steps;
+};
+
+// Visits one wide node: pushes the entered children far-to-near and returns the
+// nearest (kNone if no child is entered).
+__device__ inline float byte_f(uint32_t w, int i) { return (float)((w >> (8 * i)) & 0xFFu); }
+
+// Slab tests of one axis for 4 children packed one byte each in (near, far)
+// quantized words: t = q * scale + bias with scale = 2^e / d and
+// bias = (p - o) / d, one fused multiply-add per plane.
+__device__ inline void axis4(uint32_t qn, uint32_t qf, float scale, float bias, float* tn, float* tf) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    tn[i] = fmaf(byte_f(qn, i), scale, bias);
+    tf[i] = fmaf(byte_f(qf, i), scale, bias);
+  }
+}
+
+// Slab test of children [4h, 4h+4) of a quantized wide node (layout in bvh_build.h).
+__device__ inline void test4(uint32_t lox, uint32_t hix, uint32_t loy, uint32_t hiy, uint32_t loz, uint32_t hiz,
+                             const float* sc, const float* bs, const Ray& r, float tlo, float thi, float* dist) {
+  float tnx[4], tfx[4], tny[4], tfy[4], tnz[4], tfz[4];
+  // the near plane is the low one for a positive direction component
+  if (r.ix < 0.f) axis4(hix, lox, sc[0], bs[0], tnx, tfx); else axis4(lox, hix, sc[0], bs[0], tnx, tfx);
+  if (r.iy < 0.f) axis4(hiy, loy, sc[1], bs[1], tny, tfy); else axis4(loy, hiy, sc[1], bs[1], tny, tfy);
+  if (r.iz < 0.f) axis4(hiz, loz, sc[2], bs[2], tnz, tfz); else axis4(loz, hiz, sc[2], bs[2], tnz, tfz);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float en = fmaxf(fmaxf(tnx[i], tny[i]), fmaxf(tnz[i], tlo));
+    float ex = fminf(fminf(fminf(tfx[i], tfy[i]), tfz[i]) * kSlackUp, thi);
+    dist[i] = en <= ex ? en : INFINITY;
+  }
+}
+
+// Visits one wide node: pushes the entered children far-to-near and returns the
+// nearest (kNone if no child is entered).
+template <int W, int D, bool COUNT>
+__device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t node, const Ray& r, float thi,
+                                      Stack<D>& st, Counters& c) {
+  constexpr int kVec = W == 8 ? 6 : 4;  // 16-byte loads per node
+  const float4* np = nodes + (size_t)node * kVec;
+  float4 v[kVec];
+#pragma unroll
+  for (int k = 0; k < kVec; ++k) v[k] = gload4(np + k);
+  if (COUNT) c.nodes++;
+  const float tlo = r.t_near * kSlackDown;
+  const uint32_t ex = __float_as_uint(v[0].w);
+  float sc[3], bs[3];
+  sc[0] = __uint_as_float((ex & 0xFFu) << 23) * r.ix;
+  sc[1] = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.iy;
+  sc[2] = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.iz;
+  bs[0] = (v[0].x - r.ox) * r.ix;
+  bs[1] = (v[0].y - r.oy) * r.iy;
+  bs[2] = (v[0].z - r.oz) * r.iz;
+  float dist[W];
+  uint32_t ref[W];
+#define U(x) __float_as_uint(x)
+  if (W == 4) {
+    test4(U(v[1].x), U(v[1].y), U(v[1].z), U(v[1].w), U(v[2].x), U(v[2].y), sc, bs, r, tlo, thi, dist);
+    ref[0] = U(v[2].z), ref[1] = U(v[2].w), ref[2] = U(v[3].x), ref[3] = U(v[3].y);
+  } else {
+    // words: x = v[1] (lo0, lo1, hi0, hi1), y = v[2], z = v[3]
+    test4(U(v[1].x), U(v[1].z), U(v[2].x), U(v[2].z), U(v[3].x), U(v[3].z), sc, bs, r, tlo, thi, dist);
+    test4(U(v[1].y), U(v[1].w), U(v[2].y), U(v[2].w), U(v[3].y), U(v[3].w), sc, bs, r, tlo, thi, dist + 4);
+    ref[0] = U(v[4].x), ref[1] = U(v[4].y), ref[2] = U(v[4].z), ref[3] = U(v[4].w);
+    ref[4] = U(v[5].x), ref[5] = U(v[5].y), ref[6] = U(v[5].z), ref[7] = U(v[5].w);
+  }
+#undef U
+#define J3D_CSWAP(a, bb)           \
+  if (dist[bb] < dist[a]) {         \
+    float td = dist[a];             \
+    dist[a] = dist[bb];             \
+    dist[bb] = td;                  \
+    uint32_t tr = ref[a];           \
+    ref[a] = ref[bb];               \
+    ref[bb] = tr;                   \
+  }
+  if (W == 4) {
+    J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(1, 2)
+  } else {
+    J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(4, 6) J3D_CSWAP(5, 7)
+    J3D_CSWAP(0, 4) J3D_CSWAP(1, 5) J3D_CSWAP(2, 6) J3D_CSWAP(3, 7)
+    J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(4, 5) J3D_CSWAP(6, 7)
+    J3D_CSWAP(2, 4) J3D_CSWAP(3, 5)
+    J3D_CSWAP(1, 4) J3D_CSWAP(3, 6)
+    J3D_CSWAP(1, 2) J3D_CSWAP(3, 4) J3D_CSWAP(5, 6)
+  }
+#undef J3D_CSWAP
+#pragma unroll
+  for (int i = W - 1; i >= 1; --i)
+    if (dist[i] != INFINITY) st.push(ref[i], dist[i]);
+  return dist[0] != INFINITY ? ref[0] : kNone;
+}
+
+// Traversal state of one lane inside one object.
+struct Trav {
+  Ray r;
+  const float4* nodes;
+  const float4* tris;
+  float tmax;   // acceptance bound: min(t_far, best t)
+  float thi;    // culling bound: tmax * (1 + 2^-16)
+  uint32_t cur; // next node / leaf reference, kNone when the stack must be popped
+  uint32_t obj; // object index
+  bool any;     // any-hit (shadow) mode
+};
+
+// Starts traversal of object `oi` for the world ray (o, d) over [tn, tf].
+__device__ inline void begin_object(Trav& s, const DevObject* __restrict__ objs, uint32_t oi, f4 o, f4 d, float tn,
+                                    float tf, const Hit& best) {
+  const DevObject& ob = objs[oi];
+  s.r = make_object_ray(ob.inv_cs, o, d, tn, tf);
+  s.nodes = ob.nodes;
+  s.tris = ob.tris;
+  s.obj = oi;
+  s.tmax = best.found ? fminf(best.t, tf) : tf;
+  s.thi = s.tmax * kSlackUp;
+  s.cur = 0;  // the root is always an internal node
+}
+
+// One traversal step: visits s.cur (a wide node or a leaf), then advances
+// s.cur, popping (with culling against the current bound) if needed.
+// Returns true when this object's traversal is complete (or, in any-hit mode,
+// as soon as a hit is found).
+template <int W, int D, bool COUNT>
+__device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
+  if (COUNT) c.steps++;
+  if (s.cur & kLeafBit) {
+    // all of the leaf's triangles are known up front: load them two at a time
+    const uint32_t first = s.cur & kLeafIndexMask;
+    const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
+    for (uint32_t k = 0; k < count; k += 2) {
+      const float4* tp = s.tris + (size_t)(first + k) * 3;
+      const bool two = k + 1 < count;
+      float4 a0 = gload4(tp), b0 = gload4(tp + 1), c0 = gload4(tp + 2);
+      float4 a1 = a0, b1 = b0, c1 = c0;
+      if (two) a1 = gload4(tp + 3), b1 = gload4(tp + 4), c1 = gload4(tp + 5);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (j == 1 && !two) break;
+        const float4 a = j ? a1 : a0, e1 = j ? b1 : b0, e2 = j ? c1 : c0;
+        if (COUNT) c.tris++;
+        float t, u, v;
+        if (intersect_triangle(s.r.ox, s.r.oy, s.r.oz, s.r.dx, s.r.dy, s.r.dz, a.x, a.y, a.z, e1.x, e1.y, e1.z,
+                               e2.x, e2.y, e2.z, s.r.t_near, s.tmax, t, u, v)) {
+          uint32_t id = __float_as_uint(a.w);
+          if (s.any) {
+            best.found = true;
+            best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+            return true;
+          }
+          bool better = !best.found || t < best.t ||
+                        (t == best.t && (s.obj < best.obj || (s.obj == best.obj && id < best.tri)));
+          if (better) {
+            best.found = true;
+            best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+            s.tmax = t;
+            s.thi = t * kSlackUp;
+          }
+        }
+      }
+    }
+    s.cur = kNone;
+  } else {
+    s.cur = visit_node<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
+  }
+  if (s.cur == kNone) {
+    uint32_t ref;
+    float dist;
+    while (st.pop(ref, dist)) {
+      if (dist <= s.thi) {
+        s.cur = ref;
+        return false;
+      }
+    }
+    return true;
+  }
+  return false;
+}
+
+// Wave helpers
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ inline uint32_t rank_in_mask(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave-local pool of ray ids taken from a global atomic queue in batches.
+struct RayPool {
+  uint32_t next, end;
+  bool exhausted;
+  __device__ void init() {
+    next = end = 0;
+    exhausted = false;
+  }
+  // Hands ray ids to the lanes with want == true (all lanes must call).
+  // Returns this lane's id, or kNone.
+  __device__ uint32_t take(bool want, uint32_t* queue, uint32_t batch, uint32_t n_rays) {
+    uint32_t id = kNone;
+    uint64_t m = __ballot(want);
+    while (m && !exhausted) {
+      if (next >= end) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(queue, batch);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+        if (base >= n_rays) {
+          exhausted = true;
+          break;
+        }
+        next = base;
+        end = min(base + batch, n_rays);
+      }
+      uint32_t avail = end - next;
+      uint32_t r = rank_in_mask(m);
+      bool mine = ((m >> lane_id()) & 1ull) && r < avail;
+      if (mine) id = next + r;
+      uint32_t n = (uint32_t)__popcll(m);
+      uint32_t used = n < avail ? n : avail;
+      next += used;
+      m &= ~__ballot(mine);
+    }
+    return id;
+  }
+};
+
+}  // namespace j3d

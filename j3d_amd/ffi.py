@@ -64,7 +64,7 @@ class MeshInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("stack_depth", C.c_uint32), ("rays_per_item", C.c_uint32),
-                ("persistent", C.c_uint32), ("block_size", C.c_uint32)]
+                ("persistent", C.c_uint32), ("block_size", C.c_uint32), ("refill_min", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -72,8 +72,12 @@ class Stats(C.Structure):
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("hits", C.c_uint64), ("shadowed", C.c_uint64),
                 ("learned_resolved", C.c_uint64), ("learned_fallback", C.c_uint64),
-                ("stack_overflow", C.c_uint64),
-                ("trace_ms", C.c_float), ("shade_ms", C.c_float), ("mlp_ms", C.c_float)]
+                ("stack_overflow", C.c_uint64), ("wave_trips", C.c_uint64), ("lane_steps", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("shadow_wave_trips", C.c_uint64), ("shadow_lane_steps", C.c_uint64),
+                ("max_ray_steps", C.c_uint64), ("max_shadow_ray_steps", C.c_uint64),
+                ("ray_steps_log2", C.c_uint64 * 7), ("shadow_ray_steps_log2", C.c_uint64 * 7),
+                ("trace_ms", C.c_float), ("shade_ms", C.c_float), ("mlp_ms", C.c_float), ("shadow_ms", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -101,6 +105,8 @@ SIGNATURES = {
     "j3dg_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "j3dg_set_counting": (_I, [_P, _I]),
     "j3dg_get_stats": (_I, [_P, C.POINTER(Stats)]),
+    "j3dg_reset_timings": (_I, [_P]),
+    "j3dg_get_timings": (_I, [_P, _P, _P, _P, _P, _U32, C.POINTER(_U32)]),
     "j3dg_upload_mesh": (_I, [_P, _U32, _P, _U64, _P, _U64, _P, C.POINTER(BuildOpts), C.POINTER(_U32)]),
     "j3dg_remove_mesh": (_I, [_P, _U32]),
     "j3dg_set_mesh_cs": (_I, [_P, _U32, _P]),
@@ -116,6 +122,7 @@ SIGNATURES = {
     "j3dg_render_batch": (_I, [_P, C.POINTER(FrameDesc), _P, _U32]),
     "j3dg_trace_rays": (_I, [_P, _P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     "j3dg_any_hit_rays": (_I, [_P, _P, _U32, _P, _U64, _P]),
+    "j3dg_calibration_copy": (_I, [_P, _U64, _U32, C.POINTER(_F)]),
     # j3dg_host.h
     "j3dg_default_camera": (None, [C.POINTER(Camera)]),
     "j3dg_frustum": (None, [_F, _F, _F, _F, _F, _F, _P]),
