@@ -43,13 +43,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k"])
-    ap.add_argument("--width", type=int, default=4, choices=[4, 8])
+    ap.add_argument("--width", type=int, default=8, choices=[4, 8])
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--persistent", type=int, default=-1)
+    ap.add_argument("--coop-tail", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=0, help="rows in the CPU baseline band (0 = auto)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline time")
     ap.add_argument("--profile-only", action="store_true", help="no CPU baseline, short output (for rocprofv3)")
+    ap.add_argument("--rows", type=int, default=0, help="render only a centred band of this many rows")
     return ap.parse_args()
 
 
@@ -93,8 +94,10 @@ def read_traffic(workload_name):
         return None
 
 
-def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, rows):
-    """Times the oracle's update_canvas + canvas_to_image on a band of rows."""
+def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, min_seconds):
+    """Times the oracle's update_canvas + canvas_to_image (the reference's
+    render_scene + canvas_to_image, row-parallel over std::thread) on full
+    frames of the same workload, repeated for at least `min_seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = os.cpu_count() or 1
@@ -106,33 +109,33 @@ def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, rows):
     m = O.Mesh(v, t, db_id=(1 << 29), threads=threads)
     bvh_s = time.time() - t0
     h, w = frame.height, frame.width
-    y0 = max(0, h // 2 - rows // 2)
-    y1 = min(h - 1, y0 + rows - 1)
-    import copy
-    fr = copy.copy(frame)
-    fr.rect = (0, y0, w - 1, y1)
-    g = np.zeros((h, w), O.PIXEL_DTYPE)
-    g["object_id"] = 0xFFFFFFFF
-    t0 = time.time()
-    g, st = O.update_canvas([m], fr, gbuf=g, threads=threads)
-    im = O.canvas_to_image(g, fr, mc, cav)
-    dt = time.time() - t0
-    rays = st["primary_rays"] + st["shadow_rays"]
-    band = slice(y0, y1 + 1)
-    mism = int((g["object_id"][band] != gpu_gbuf["object_id"][band]).sum() +
-               (g["depth"][band].view(np.uint32) != gpu_gbuf["depth"][band].view(np.uint32)).sum() +
-               (g["mark"][band] != gpu_gbuf["mark"][band]).sum())
-    # edge pixels of the band's first row see an up neighbour outside the band
-    d = np.abs(im[y0 + 1:y1 + 1].view(np.uint8).astype(int) - gpu_im[y0 + 1:y1 + 1].view(np.uint8).astype(int))
+    frames, rays, dt = 0, 0, 0.0
+    g = None
+    while dt < min_seconds or frames == 0:
+        g = np.zeros((h, w), O.PIXEL_DTYPE)
+        g["object_id"] = 0xFFFFFFFF
+        t0 = time.time()
+        g, st = O.update_canvas([m], frame, gbuf=g, threads=threads)
+        im = O.canvas_to_image(g, frame, mc, cav)
+        dt += time.time() - t0
+        rays += st["primary_rays"] + st["shadow_rays"]
+        frames += 1
+        if frames >= 500:
+            break
+    mism = int((g["object_id"] != gpu_gbuf["object_id"]).sum() +
+               (g["depth"].view(np.uint32) != gpu_gbuf["depth"].view(np.uint32)).sum() +
+               (g["mark"] != gpu_gbuf["mark"]).sum())
+    d = np.abs(im.view(np.uint8).astype(int) - gpu_im.view(np.uint8).astype(int))
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
     except Exception:
         model = "unknown"
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "rows %d-%d of the %dx%d frame: %d primary + %d shadow rays, update_canvas + canvas_to_image "
-                      "in %.2fs (oracle BVH build %.1fs excluded)" % (y0, y1, w, h, st["primary_rays"],
-                                                                       st["shadow_rays"], dt, bvh_s),
-            "cpu_model": model, "gbuffer_mismatches_vs_gpu": mism, "rgba_max_diff_vs_gpu": int(d.max())}
+            "sample": "%d full %dx%d frames (%d primary + %d shadow rays each), update_canvas + canvas_to_image, "
+                      "%.1fs of CPU time (oracle BVH build %.1fs excluded)" % (frames, w, h, st["primary_rays"],
+                                                                             st["shadow_rays"], dt, bvh_s),
+            "cpu_model": model, "gbuffer_mismatches_vs_gpu": mism, "rgba_max_diff_vs_gpu": int(d.max()),
+            "ms_per_frame": dt / frames * 1e3}
 
 
 def main():
@@ -151,8 +154,8 @@ def main():
         tun["stack_depth"] = a.stack_depth
     if a.block:
         tun["block_size"] = a.block
-    if a.persistent >= 0:
-        tun["persistent"] = a.persistent
+    if a.coop_tail >= 0:
+        tun["coop_tail"] = a.coop_tail
     r.set_tuning(**tun)
     t0 = time.time()
     slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width)
@@ -162,7 +165,11 @@ def main():
         info["device_bytes"] / 1e9))
     r.set_matcap(mc, cav)
     tg = r.create_target(w, h)
-    fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+    rect = None
+    if a.rows:
+        y0 = max(0, h // 2 - a.rows // 2)
+        rect = (0, y0, w - 1, min(h - 1, y0 + a.rows - 1))
+    fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light, rect=rect)
 
     # counters (node visits, triangle tests) from one counting frame
     r.set_counting(True)
@@ -188,7 +195,7 @@ def main():
     ms_per_step = wall / a.steps * 1e3
     value = rays_per_frame * a.steps / wall / 1e6
 
-    node_bytes = 32 * a.width
+    node_bytes = 96 if a.width == 8 else 64  # quantized wide node
     npx = w * h
     # primary k_trace launch: node and triangle reads + 20 B of G-buffer writes per pixel
     algo_bytes = st["node_visits"] * node_bytes + st["tri_tests"] * 48 + 20 * npx
@@ -218,9 +225,8 @@ def main():
            "tuning": r.get_tuning()}
 
     if not a.no_cpu_baseline and not a.profile_only:
-        rows = a.cpu_rows or max(8, min(h, 216))
-        log("cpu baseline on %d rows ..." % rows)
-        out["cpu_baseline"] = cpu_baseline(v, t, fr, mc, cav, gpu_g, gpu_im, rows)
+        log("cpu baseline for >= %.0fs ..." % a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(v, t, fr, mc, cav, gpu_g, gpu_im, a.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     r.close()

@@ -90,7 +90,7 @@ typedef struct j3dg_frame_desc {
 
 /* BVH build options for j3dg_upload_mesh (0 = default). */
 typedef struct j3dg_build_opts {
-  uint32_t width;     /* 4 (default) or 8 children per node */
+  uint32_t width;     /* 4 or 8 (default) children per node */
   uint32_t leaf_size; /* max triangles per leaf, default 4 */
   uint32_t threads;   /* host build threads, 0 = hardware_concurrency */
 } j3dg_build_opts;
@@ -108,7 +108,7 @@ typedef struct j3dg_mesh_info {
 typedef struct j3dg_tuning {
   uint32_t stack_depth;    /* LDS stack entries per lane: 8, 16 or 32 */
   uint32_t rays_per_item;  /* rays each lane takes per queue fetch: 1, 2 or 4 */
-  uint32_t persistent;     /* reserved (waves are always persistent on an atomic ray queue) */
+  uint32_t coop_tail;      /* 1 = once a wave's rays run out, its last <= 8 rays get 8 lanes each */
   uint32_t block_size;     /* threads per workgroup: 64, 128 or 256 */
   uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves) */
 } j3dg_tuning;
@@ -201,6 +201,11 @@ int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
    repeated `repeats` times: the reference point that scales profiler byte
    counters to HBM bytes.  *ms receives the mean duration of one copy. */
 int j3dg_calibration_copy(j3dg_ctx* ctx, uint64_t bytes, uint32_t repeats, float* ms);
+/* With counting on, the primary trace kernel of the last render records one
+   row of 8 uint64 per wave: shader clock at start, when its ray pool ran dry,
+   when it entered the cooperative tail mode (0 = never), at exit; loop trips,
+   trips in tail mode, CU id, 0. */
+int j3dg_debug_timeline(j3dg_ctx* ctx, uint64_t* out, uint32_t max_waves, uint32_t* n_waves);
 
 #ifdef __cplusplus
 }

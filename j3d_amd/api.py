@@ -226,7 +226,7 @@ class Renderer:
     def _c(self, rc, what):
         _check(self.lib, self.ctx, rc, what)
 
-    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=4, leaf_size=4, threads=0):
+    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=8, leaf_size=4, threads=0):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
         o = ffi.BuildOpts(int(width), int(leaf_size), int(threads))
@@ -301,8 +301,8 @@ class Renderer:
     def synchronize(self):
         self._c(self.lib.j3dg_synchronize(self.ctx), "j3dg_synchronize")
 
-    def set_tuning(self, stack_depth=16, rays_per_item=1, persistent=1, block_size=256, refill_min=16):
-        t = ffi.Tuning(stack_depth, rays_per_item, persistent, block_size, refill_min)
+    def set_tuning(self, stack_depth=16, rays_per_item=1, coop_tail=1, block_size=256, refill_min=8):
+        t = ffi.Tuning(stack_depth, rays_per_item, coop_tail, block_size, refill_min)
         self._c(self.lib.j3dg_set_tuning(self.ctx, C.byref(t)), "j3dg_set_tuning")
 
     def get_tuning(self):
@@ -336,6 +336,13 @@ class Renderer:
                                           C.byref(n)), "j3dg_get_timings")
         k = n.value
         return {"trace_ms": tr[:k], "shade_ms": sh[:k], "mlp_ms": ml[:k], "shadow_ms": sd[:k]}
+
+    def debug_timeline(self, max_waves=1 << 16):
+        """Per-wave timeline of the last counting render (see j3dg_debug_timeline)."""
+        out = np.zeros((max_waves, 8), np.uint64)
+        n = C.c_uint32(0)
+        self._c(self.lib.j3dg_debug_timeline(self.ctx, ffi.ptr(out), max_waves, C.byref(n)), "j3dg_debug_timeline")
+        return out[:n.value]
 
     def calibration_copy(self, nbytes, repeats=1):
         ms = C.c_float(0)

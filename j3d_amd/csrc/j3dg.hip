@@ -63,7 +63,8 @@ template <int W, int D, bool COUNT, int MODE>
 __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32_t n_rays_q,
                                                const DevObject* __restrict__ objs, uint32_t n_obj, uint2* spill,
                                                uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
-                                               uint32_t batch, uint32_t refill_min) {
+                                               uint32_t batch, uint32_t refill_min, uint32_t coop_tail,
+                                               uint64_t* timeline) {
   extern __shared__ uint2 lds_stack[];
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
@@ -71,6 +72,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   uint32_t acc[5] = {0, 0, 0, 0, 0};  // wave-uniform event counts
   uint32_t trips = 0;
   uint32_t ray_steps = 0;            // counting: steps of the current ray
+  uint32_t max_steps = 0, hist[7] = {0, 0, 0, 0, 0, 0, 0};
+  bool coop = false;                 // cooperative tail mode (wave-uniform)
+  // counting: per-wave timeline (shader clock at start, pool exhausted, tail mode, end; trips)
+  uint64_t tl_start = COUNT ? (uint64_t)clock64() : 0, tl_exh = 0, tl_coop = 0;
+  uint32_t trips_coop = 0;
   RayPool pool;
   pool.init();
   uint32_t n_rays = n_rays_q;
@@ -154,36 +160,65 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
     }
+    if ((MODE == kModeFrame) && __ballot(ev & kEvPrimary)) acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
+    ev &= ~kEvPrimary;
+    if (COUNT && pool.exhausted && !tl_exh) tl_exh = (uint64_t)clock64();
+    if (coop_tail && !coop && pool.exhausted) {
+      const uint64_t act = __ballot(phase != 0);
+      const uint32_t n_act = (uint32_t)__popcll(act);
+      if (n_act > 0 && n_act <= 8) {
+        // hand each remaining ray to a group of 8 lanes (group g: the g-th active lane)
+        const uint32_t g = lane_id() >> 3;
+        uint64_t m = act;
+        for (uint32_t i = 0; i < g && m; ++i) m &= m - 1;
+        const bool has = g < n_act;
+        const int leader = has ? (int)__builtin_ctzll(m) : (int)lane_id();
+        shfl_from(phase, leader);
+        shfl_from(pix, leader);
+        shfl_from(ray_steps, leader);
+        shfl_from(wo, leader);
+        shfl_from(wd, leader);
+        shfl_from(s, leader);
+        shfl_from(best, leader);
+        shfl_from(st, leader);
+        if (!has) phase = 0u;
+        coop = true;
+        if (COUNT) tl_coop = (uint64_t)clock64();
+      }
+    }
+    const bool writer = !coop || group_lane() == 0;
     if (COUNT) trips++;
+    if (COUNT && coop) trips_coop++;
     if (phase != 0) {
       if (COUNT) ray_steps++;
-      bool done = n_obj == 0 || trav_step<W, D, COUNT>(s, best, st, c);
+      bool done = n_obj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, c) : trav_step<W, D, COUNT>(s, best, st, c));
       if (done && !(any_mode && best.found) && s.obj + 1 < n_obj) {
         begin_object(s, objs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
         done = false;
       }
-      if (done && COUNT && phase != 1) {
-        // per-ray step histogram: slot 16/17 = max steps (primary/shadow), 18.. = log2 buckets
-        const int hb = MODE == kModeShadow ? 25 : 18;
-        atomicMax((unsigned long long*)&counters[MODE == kModeShadow ? 17 : 16], (unsigned long long)ray_steps);
-        uint32_t b = 31u - __clz(ray_steps | 1u);
-        atomicAdd((unsigned long long*)&counters[hb + (b > 6 ? 6 : b)], 1ull);
+      if (done && COUNT && phase != 1 && writer) {
+        // per-ray step histogram, kept in registers and reduced once at exit
+        max_steps = max(max_steps, ray_steps);
+        const uint32_t b = 31u - __clz(ray_steps | 1u);
+        hist[b > 6 ? 6 : b]++;
       }
       if (done) {
         if (MODE == kModeAnyRays) {
-          q.any[pix] = best.found ? 1 : 0;
+          if (writer) q.any[pix] = best.found ? 1 : 0;
           phase = 0;
         } else if (MODE == kModeRays) {
-          q.obj[pix] = best.found ? best.obj : kMiss;
-          q.tri[pix] = best.found ? best.tri : kMiss;
-          q.t[pix] = best.found ? best.t : FLT_MAX;
-          q.u[pix] = best.found ? best.u : 0.f;
-          q.v[pix] = best.found ? best.v : 0.f;
+          if (writer) {
+            q.obj[pix] = best.found ? best.obj : kMiss;
+            q.tri[pix] = best.found ? best.tri : kMiss;
+            q.t[pix] = best.found ? best.t : FLT_MAX;
+            q.u[pix] = best.found ? best.u : 0.f;
+            q.v[pix] = best.found ? best.v : 0.f;
+          }
           phase = 0;
         } else if (MODE == kModeShadow) {
           if (best.found) {
-            *(reinterpret_cast<uint8_t*>(f.gbuf) + (size_t)pix * 32) = 1;
+            if (writer) *(reinterpret_cast<uint8_t*>(f.gbuf) + (size_t)pix * 32) = 1;
             ev |= kEvShadowed;
           }
           phase = 0;
@@ -201,12 +236,14 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
           if (best.found) {
             ev |= kEvHit;
-            pf[3] = best.t;
-            pu[4] = best.tri;
-            pf[5] = best.u;
-            pf[6] = best.v;
-            pu[7] = best.obj;  // object index; k_finish replaces it with the db_id
-          } else {
+            if (writer) {
+              pf[3] = best.t;
+              pu[4] = best.tri;
+              pf[5] = best.u;
+              pf[6] = best.v;
+              pu[7] = best.obj;  // object index; k_finish replaces it with the db_id
+            }
+          } else if (writer) {
             pf[1] = 0.f;
             pf[2] = 0.f;
             pf[3] = FLT_MAX;
@@ -216,6 +253,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
     }
+    if (!writer) ev = 0;
     if ((MODE == kModeFrame || MODE == kModeShadow) && __ballot(ev != 0)) {
       acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
       acc[1] += (uint32_t)__popcll(__ballot(ev & kEvHit));
@@ -237,8 +275,20 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     add_counter(counters, cb, c.nodes);
     add_counter(counters, cb + 1, c.tris);
     add_counter(counters, MODE == kModeShadow ? 15 : 10, c.steps);
+    // per-ray step histogram: slot 16/17 = max steps (primary/shadow), 18.. / 25.. = log2 buckets
+    const int hb = MODE == kModeShadow ? 25 : 18;
+    for (int k = 0; k < 7; ++k) add_counter(counters, hb + k, hist[k]);
+    uint32_t mx = max_steps;
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+    if (lane_id() == 0) atomicMax((unsigned long long*)&counters[MODE == kModeShadow ? 17 : 16], (unsigned long long)mx);
   }
   if (__any(st.overflow) && lane_id() == 0) atomicAdd((unsigned long long*)&counters[6], 1ull);
+  if (COUNT && timeline && lane_id() == 0) {
+    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t* o = timeline + 8 * w;
+    o[0] = tl_start, o[1] = tl_exh, o[2] = tl_coop, o[3] = (uint64_t)clock64();
+    o[4] = trips, o[5] = trips_coop, o[6] = (uint64_t)__smid(), o[7] = 0;
+  }
 }
 
 // Completes the hit pixels of the rectangle (j3d/canvas.cpp:788-801, 836-853):
@@ -546,8 +596,22 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
   uint32_t nr = n_rays, no = n_obj;
   uint32_t batch = 64u * tuning.rays_per_item;
   uint32_t refill = tuning.refill_min;
+  uint32_t coop = tuning.coop_tail;
   const DevObject* objs = d_objs;
-  void* args[] = {&f, &rq, &nr, &objs, &no, &sp, &cap, &qu, &cnt, &batch, &refill};
+  uint64_t* tl = nullptr;
+  if (counting && mode == kModeFrame) {
+    uint64_t waves = lanes / 64;
+    if (waves * 8 > timeline_capacity) {
+      if (d_timeline) hipFree(d_timeline);
+      d_timeline = nullptr;
+      timeline_capacity = 0;
+      HIP_TRY(hipMalloc(&d_timeline, waves * 8 * sizeof(uint64_t)), J3DG_ERR_OUT_OF_MEMORY);
+      timeline_capacity = waves * 8;
+    }
+    tl = d_timeline;
+    timeline_waves = (uint32_t)waves;
+  }
+  void* args[] = {&f, &rq, &nr, &objs, &no, &sp, &cap, &qu, &cnt, &batch, &refill, &coop, &tl};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;
 }
@@ -707,7 +771,7 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
   if (n.stack_depth == 0) n.stack_depth = 16;
   if (n.rays_per_item == 0) n.rays_per_item = 1;
   if (n.block_size == 0) n.block_size = 256;
-  if (n.refill_min == 0) n.refill_min = 16;
+  if (n.refill_min == 0) n.refill_min = 8;
   if (n.refill_min > 64) return ctx->fail(J3DG_ERR_INVALID_ARG, "refill_min must be in [1, 64]");
   if (n.stack_depth != 8 && n.stack_depth != 16 && n.stack_depth != 32)
     return ctx->fail(J3DG_ERR_INVALID_ARG, "stack_depth must be 8, 16 or 32");
@@ -811,6 +875,7 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   auto t0 = std::chrono::steady_clock::now();
   BuildOptions bo;
+  bo.width = 8;
   if (opts) {
     if (opts->width) bo.width = (int)opts->width;
     if (opts->leaf_size) bo.leaf_size = (int)opts->leaf_size;
@@ -1194,6 +1259,18 @@ int j3dg_calibration_copy(j3dg_ctx* h, uint64_t bytes, uint32_t repeats, float* 
   hipFree(a);
   hipFree(b);
   if (err != hipSuccess) return ctx->fail(J3DG_ERR_LAUNCH, hipGetErrorString(err));
+  return J3DG_OK;
+}
+
+int j3dg_debug_timeline(j3dg_ctx* h, uint64_t* out, uint32_t max_waves, uint32_t* n_waves) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (!n_waves) return ctx->fail(J3DG_ERR_INVALID_ARG, "null count");
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  uint32_t n = std::min(max_waves, ctx->timeline_waves);
+  if (n && out && ctx->d_timeline)
+    HIP_CALL(hipMemcpy(out, ctx->d_timeline, (size_t)n * 8 * sizeof(uint64_t), hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
+  *n_waves = n;
   return J3DG_OK;
 }
 

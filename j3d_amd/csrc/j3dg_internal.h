@@ -137,7 +137,7 @@ struct Ctx {
   hipEvent_t* cur_ev() { return ring.empty() ? ev : &ring[8 * (ring_n % kRing)]; }
   bool traced = false, shaded = false, mlp_ran = false;
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 1, 256, 16};
+  j3dg_tuning tuning{16, 1, 1, 256, 8};  // stack depth, rays per item, coop tail, block, refill
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   uint32_t* d_queue = nullptr;
@@ -160,6 +160,9 @@ struct Ctx {
   int ensure_shadow_queue(uint64_t rays);
   float4* d_shadow = nullptr;
   uint64_t shadow_capacity = 0;
+  uint64_t* d_timeline = nullptr;   // counting mode: per-wave timeline of the primary trace
+  uint64_t timeline_capacity = 0;
+  uint32_t timeline_waves = 0;
   uint32_t trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const;
   int launch_shade(const ShadeParams& sp);
   void invalidate_graphs() {}
@@ -171,6 +174,8 @@ struct Ctx {
     if (d_matcap) hipFree(d_matcap);
     if (d_shadow) hipFree(d_shadow);
     d_shadow = nullptr;
+    if (d_timeline) hipFree(d_timeline);
+    d_timeline = nullptr;
     d_queue = nullptr, d_counters = nullptr, spill = nullptr, d_objs = nullptr, d_matcap = nullptr;
   }
 };

@@ -339,4 +339,164 @@ struct RayPool {
   }
 };
 
+// ---- cooperative tail mode ---------------------------------------------------
+// Once a wave's ray pool is exhausted and at most 8 rays remain in it, the
+// remaining rays are each given a group of 8 lanes.  Every lane of a group
+// holds an identical copy of the ray's state; per step lane j decodes child j
+// of the node (or tests triangle j of the leaf) and the group combines the
+// results with cross-lane shuffles.  This cuts the per-step instruction chain
+// of the last, longest rays, which bound the kernel's duration.
+
+__device__ inline uint32_t group_lane() { return threadIdx.x & 7u; }
+__device__ inline uint32_t group_mask(uint64_t m) { return (uint32_t)(m >> (lane_id() & ~7u)) & 0xFFu; }
+
+// Replaces v by lane `src`'s v, one 32-bit word at a time (no second copy live).
+template <class T>
+__device__ inline void shfl_from(T& v, int src) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words only");
+  uint32_t* a = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) a[i] = (uint32_t)__shfl((int)a[i], src, 64);
+}
+
+// Cooperative node visit: lane j of the group handles child j.  Returns the
+// nearest entered child (identical in all lanes of the group) and pushes the
+// others far-to-near on the group's shared stack.
+template <int W, int D, bool COUNT>
+__device__ inline uint32_t coop_visit(const float4* __restrict__ nodes, uint32_t node, const Ray& r, float thi,
+                                      Stack<D>& st, Counters& c) {
+  constexpr int kVec = W == 8 ? 6 : 4;
+  const float4* np = nodes + (size_t)node * kVec;
+  const uint32_t j = group_lane();
+  const float4 h = gload4(np);
+  const float4 qx = gload4(np + 1), qy = gload4(np + 2), qz = gload4(np + 3);
+  if (COUNT && j == 0) c.nodes++;
+  const uint32_t ex = __float_as_uint(h.w);
+  const float scx = __uint_as_float((ex & 0xFFu) << 23) * r.ix;
+  const float scy = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.iy;
+  const float scz = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.iz;
+  const float bx = (h.x - r.ox) * r.ix, by = (h.y - r.oy) * r.iy, bz = (h.z - r.oz) * r.iz;
+#define U(x) __float_as_uint(x)
+  uint32_t lox, hix, loy, hiy, loz, hiz, ref;
+  const int b = (int)(j & 3u);
+  if (W == 4) {
+    lox = U(qx.x), hix = U(qx.y), loy = U(qx.z), hiy = U(qx.w), loz = U(qy.x), hiz = U(qy.y);
+    const float4 rr = gload4(np + 3);
+    ref = j == 0 ? U(qy.z) : j == 1 ? U(qy.w) : j == 2 ? U(rr.x) : U(rr.y);
+  } else {
+    const bool up = j >= 4;
+    lox = up ? U(qx.y) : U(qx.x), hix = up ? U(qx.w) : U(qx.z);
+    loy = up ? U(qy.y) : U(qy.x), hiy = up ? U(qy.w) : U(qy.z);
+    loz = up ? U(qz.y) : U(qz.x), hiz = up ? U(qz.w) : U(qz.z);
+    const float4 rr = gload4(np + (up ? 5 : 4));
+    ref = b == 0 ? U(rr.x) : b == 1 ? U(rr.y) : b == 2 ? U(rr.z) : U(rr.w);
+  }
+#undef U
+  const float tlo = r.t_near * kSlackDown;
+  const float lx = fmaf(byte_f(lox, b), scx, bx), ux = fmaf(byte_f(hix, b), scx, bx);
+  const float ly = fmaf(byte_f(loy, b), scy, by), uy = fmaf(byte_f(hiy, b), scy, by);
+  const float lz = fmaf(byte_f(loz, b), scz, bz), uz = fmaf(byte_f(hiz, b), scz, bz);
+  const float tnx = r.ix < 0.f ? ux : lx, tfx = r.ix < 0.f ? lx : ux;
+  const float tny = r.iy < 0.f ? uy : ly, tfy = r.iy < 0.f ? ly : uy;
+  const float tnz = r.iz < 0.f ? uz : lz, tfz = r.iz < 0.f ? lz : uz;
+  const float en = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tlo));
+  const float exd = fminf(fminf(fminf(tfx, tfy), tfz) * kSlackUp, thi);
+  float dist = (W == 8 || j < 4) && en <= exd ? en : INFINITY;
+  // bitonic sort of the group's 8 (dist, ref) pairs, ascending
+#pragma unroll
+  for (int k = 2; k <= 8; k <<= 1) {
+#pragma unroll
+    for (int m = k >> 1; m >= 1; m >>= 1) {
+      const float od = __shfl_xor(dist, m, 64);
+      const uint32_t orf = (uint32_t)__shfl_xor((int)ref, m, 64);
+      const bool asc = k == 8 || (j & (uint32_t)k) == 0;
+      const bool low = (j & (uint32_t)m) == 0;
+      const bool take = low == asc ? od < dist : od > dist;
+      if (take) dist = od, ref = orf;
+    }
+  }
+  const uint32_t hits = group_mask(__ballot(dist != INFINITY));
+  const int n_hit = __popc(hits);
+  const uint32_t first = (uint32_t)__shfl((int)ref, (int)(lane_id() & ~7u), 64);
+  // lanes 1..n_hit-1 push, farthest deepest: lane j goes to entry sp + n_hit - 1 - j
+  if (n_hit > 1) {
+    if (j >= 1 && (int)j < n_hit) {
+      const uint32_t e = st.sp + (uint32_t)(n_hit - 1) - j;
+      const uint32_t slot = e & (D - 1);
+      bool ok = true;
+      if (e >= (uint32_t)D) {
+        if (e - D >= st.cap) {
+          st.overflow = true;
+          ok = false;
+        } else {
+          st.spill[(uint64_t)(e - D) * st.sstride] = st.lds[slot * st.stride];
+        }
+      }
+      if (ok) st.lds[slot * st.stride] = make_uint2(ref, __float_as_uint(dist));
+    }
+    st.sp += (uint32_t)(n_hit - 1);
+  }
+  return n_hit ? first : kNone;
+}
+
+// Cooperative step: as trav_step, lanes of a group splitting the node's
+// children or the leaf's triangles.  All lanes of the group end identical.
+template <int W, int D, bool COUNT>
+__device__ inline bool coop_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
+  const uint32_t j = group_lane();
+  if (COUNT && j == 0) c.steps++;
+  if (s.cur & kLeafBit) {
+    const uint32_t first = s.cur & kLeafIndexMask;
+    const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
+    float t = INFINITY, u = 0.f, v = 0.f;
+    uint32_t id = kMiss;
+    if (j < count) {
+      const float4* tp = s.tris + (size_t)(first + j) * 3;
+      const float4 a = gload4(tp), e1 = gload4(tp + 1), e2 = gload4(tp + 2);
+      float tt, uu, vv;
+      if (intersect_triangle(s.r.ox, s.r.oy, s.r.oz, s.r.dx, s.r.dy, s.r.dz, a.x, a.y, a.z, e1.x, e1.y, e1.z, e2.x,
+                             e2.y, e2.z, s.r.t_near, s.tmax, tt, uu, vv))
+        t = tt, u = uu, v = vv, id = __float_as_uint(a.w);
+    }
+    if (COUNT && j == 0) c.tris += count;
+    // lexicographic (t, id) minimum over the group
+#pragma unroll
+    for (int m = 1; m <= 4; m <<= 1) {
+      const float ot = __shfl_xor(t, m, 64), ou = __shfl_xor(u, m, 64), ov = __shfl_xor(v, m, 64);
+      const uint32_t oid = (uint32_t)__shfl_xor((int)id, m, 64);
+      if (ot < t || (ot == t && oid < id)) t = ot, u = ou, v = ov, id = oid;
+    }
+    if (id != kMiss) {
+      if (s.any) {
+        best.found = true;
+        best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+        return true;
+      }
+      bool better = !best.found || t < best.t ||
+                    (t == best.t && (s.obj < best.obj || (s.obj == best.obj && id < best.tri)));
+      if (better) {
+        best.found = true;
+        best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+        s.tmax = t;
+        s.thi = t * kSlackUp;
+      }
+    }
+    s.cur = kNone;
+  } else {
+    s.cur = coop_visit<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
+  }
+  if (s.cur == kNone) {
+    uint32_t ref;
+    float dist;
+    while (st.pop(ref, dist)) {
+      if (dist <= s.thi) {
+        s.cur = ref;
+        return false;
+      }
+    }
+    return true;
+  }
+  return false;
+}
+
 }  // namespace j3d

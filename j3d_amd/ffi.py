@@ -64,7 +64,7 @@ class MeshInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("stack_depth", C.c_uint32), ("rays_per_item", C.c_uint32),
-                ("persistent", C.c_uint32), ("block_size", C.c_uint32), ("refill_min", C.c_uint32)]
+                ("coop_tail", C.c_uint32), ("block_size", C.c_uint32), ("refill_min", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -123,6 +123,7 @@ SIGNATURES = {
     "j3dg_trace_rays": (_I, [_P, _P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     "j3dg_any_hit_rays": (_I, [_P, _P, _U32, _P, _U64, _P]),
     "j3dg_calibration_copy": (_I, [_P, _U64, _U32, C.POINTER(_F)]),
+    "j3dg_debug_timeline": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
     # j3dg_host.h
     "j3dg_default_camera": (None, [C.POINTER(Camera)]),
     "j3dg_frustum": (None, [_F, _F, _F, _F, _F, _F, _P]),
