@@ -292,74 +292,85 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
 }
 
 // Completes the hit pixels of the rectangle (j3d/canvas.cpp:788-801, 836-853):
-// view-space normal, db_id and mark, and appends one shadow ray per hit pixel
-// to a compacted queue (wave ballot + one atomic per wave).
+// view-space normal and db_id, and appends one shadow ray per hit pixel to a
+// compacted queue.  Each block handles kFinishPerThread * 256 pixels and
+// reserves its queue slots with one atomic per pass.
+constexpr uint32_t kFinishPerThread = 4;
+
 __global__ void __launch_bounds__(256) k_finish(FrameParams f, const DevObject* __restrict__ objs,
                                                 float4* __restrict__ shadow_rays, uint32_t* shadow_count,
                                                 uint64_t* counters) {
-  const uint32_t rw = (uint32_t)(f.x1 - f.x0 + 1);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t rh = (uint32_t)(f.y1 - f.y0 + 1);
-  bool has_shadow = false;
-  f4 pos = {0.f, 0.f, 0.f, 0.f}, ld = {0.f, 0.f, 0.f, 0.f};
-  uint32_t pix = 0;
-  if (f.x1 >= f.x0 && f.y1 >= f.y0 && i < rw * rh) {
-    const uint32_t px = (uint32_t)f.x0 + i % rw, py = (uint32_t)f.y0 + i / rw;
-    pix = py * f.w + px;
-    char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
-    uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
-    float* pf = reinterpret_cast<float*>(pp);
-    const uint32_t tri = pu[4];
-    if (tri != kMiss) {
-      const uint32_t oi = pu[7];
-      const DevObject& ob = objs[oi];
-      const float* tnrm = ob.normals + 3 * (size_t)tri;
-      f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
-      n = mat_vec(f.cs_inv, n);
-      n = mat_vec(ob.cs, n);
-      *reinterpret_cast<uint8_t*>(pp) = 0;
-      pf[1] = n.x;
-      pf[2] = n.y;
-      pu[7] = ob.db_id;
-      if (f.flags & J3DG_SHADOW) {
-        const float bu = pf[5], bv = pf[6];
-        const uint32_t* tr = ob.idx + 3 * (size_t)tri;
-        f4 V[3];
+  __shared__ uint32_t wave_n[4], block_base;
+  const uint32_t rw = f.x1 >= f.x0 ? (uint32_t)(f.x1 - f.x0 + 1) : 0u;
+  const uint32_t rh = f.y1 >= f.y0 ? (uint32_t)(f.y1 - f.y0 + 1) : 0u;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t total = 0;
+  for (uint32_t it = 0; it < kFinishPerThread; ++it) {
+    const uint32_t i = (blockIdx.x * kFinishPerThread + it) * blockDim.x + threadIdx.x;
+    bool has_shadow = false;
+    f4 pos = {0.f, 0.f, 0.f, 0.f}, ld = {0.f, 0.f, 0.f, 0.f};
+    uint32_t pix = 0;
+    if (i < rw * rh) {
+      const uint32_t px = (uint32_t)f.x0 + i % rw, py = (uint32_t)f.y0 + i / rw;
+      pix = py * f.w + px;
+      char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
+      uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
+      float* pf = reinterpret_cast<float*>(pp);
+      const uint32_t tri = pu[4];
+      if (tri != kMiss) {
+        const uint32_t oi = pu[7];
+        const DevObject& ob = objs[oi];
+        const float* tnrm = ob.normals + 3 * (size_t)tri;
+        f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
+        n = mat_vec(f.cs_inv, n);
+        n = mat_vec(ob.cs, n);
+        *reinterpret_cast<uint8_t*>(pp) = 0;  // mark (j3d/canvas.cpp:801)
+        pf[1] = n.x;
+        pf[2] = n.y;
+        pu[7] = ob.db_id;
+        if (f.flags & J3DG_SHADOW) {
+          const float bu = pf[5], bv = pf[6];
+          const uint32_t* tr = ob.idx + 3 * (size_t)tri;
+          f4 V[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float* vv = ob.verts + 3 * (size_t)gload(tr + k);
-          V[k] = mat_vec(ob.cs, f4{gload(vv), gload(vv + 1), gload(vv + 2), 1.f});
+          for (int k = 0; k < 3; ++k) {
+            const float* vv = ob.verts + 3 * (size_t)gload(tr + k);
+            V[k] = mat_vec(ob.cs, f4{gload(vv), gload(vv + 1), gload(vv + 2), 1.f});
+          }
+          float s0 = 1.f - bu - bv;
+          pos = f4{V[0].x * s0 + bu * V[1].x + bv * V[2].x, V[0].y * s0 + bu * V[1].y + bv * V[2].y,
+                   V[0].z * s0 + bu * V[1].z + bv * V[2].z, V[0].w * s0 + bu * V[1].w + bv * V[2].w};
+          if (f.flags & J3DG_DIRECTIONAL_LIGHT)
+            ld = f4{f.light_dir[0], f.light_dir[1], f.light_dir[2], 0.f};
+          else
+            ld = f4{f.light[0] - pos.x, f.light[1] - pos.y, f.light[2] - pos.z, f.light[3] - pos.w};
+          has_shadow = true;
         }
-        float s0 = 1.f - bu - bv;
-        pos = f4{V[0].x * s0 + bu * V[1].x + bv * V[2].x, V[0].y * s0 + bu * V[1].y + bv * V[2].y,
-                 V[0].z * s0 + bu * V[1].z + bv * V[2].z, V[0].w * s0 + bu * V[1].w + bv * V[2].w};
-        if (f.flags & J3DG_DIRECTIONAL_LIGHT)
-          ld = f4{f.light_dir[0], f.light_dir[1], f.light_dir[2], 0.f};
-        else
-          ld = f4{f.light[0] - pos.x, f.light[1] - pos.y, f.light[2] - pos.z, f.light[3] - pos.w};
-        has_shadow = true;
       }
     }
-  }
-  uint64_t m = __ballot(has_shadow);
-  if (m) {
-    uint32_t base = 0;
-    if (lane_id() == 0) {
-      base = atomicAdd(shadow_count, (uint32_t)__popcll(m));
-      atomicAdd((unsigned long long*)&counters[2], (unsigned long long)__popcll(m));
+    if (!(f.flags & J3DG_SHADOW)) continue;  // uniform
+    const uint64_t m = __ballot(has_shadow);
+    if (lane_id() == 0) wave_n[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+      block_base = n ? atomicAdd(shadow_count, n) : 0u;
+      total += n;
     }
-    base = __shfl(base, 0, 64);
+    __syncthreads();
+    uint32_t base = block_base;
+    for (uint32_t k = 0; k < wave; ++k) base += wave_n[k];
     if (has_shadow) {
       float4* r = shadow_rays + 3 * (size_t)(base + rank_in_mask(m));
       r[0] = make_float4(pos.x, pos.y, pos.z, pos.w);
       r[1] = make_float4(ld.x, ld.y, ld.z, ld.w);
       r[2] = make_float4(__uint_as_float(pix), 0.f, 0.f, 0.f);
     }
+    __syncthreads();
   }
+  if (threadIdx.x == 0 && total) atomicAdd((unsigned long long*)&counters[2], (unsigned long long)total);
 }
 
-// Clears the ray queues and counters of a launch (a kernel rather than a
-// memset so that it can live in a captured HIP graph).
 __global__ void k_reset(uint32_t* queue, uint64_t* counters, uint32_t n_counters) {
   uint32_t i = threadIdx.x;
   if (i < 16) queue[i] = 0u;
@@ -651,8 +662,9 @@ int Ctx::launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool
     uint32_t rh = fp.y1 >= fp.y0 ? (uint32_t)(fp.y1 - fp.y0 + 1) : 0;
     uint32_t n = rw * rh;
     if (n) {
-      hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, stream, fp, (const DevObject*)d_objs,
-                         d_shadow, d_queue + 1, d_counters);
+      const uint32_t per_block = 256u * kFinishPerThread;
+      hipLaunchKernelGGL(k_finish, dim3((n + per_block - 1) / per_block), dim3(256), 0, stream, fp,
+                         (const DevObject*)d_objs, d_shadow, d_queue + 1, d_counters);
       HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
     }
     if (shadow && n) {
