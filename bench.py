@@ -4,11 +4,15 @@
 Workload (default): seeded displaced torus, 5000 x 2800 grid = 14,000,000
 vertices / 28,000,000 triangles; orbit camera at distance 3, 60 degree vertical
 fov, 1920 x 1080 primary rays plus one directional shadow ray per hit pixel;
-edges + matcap shading.  A step is one frame: raygen + BVH traversal (+ the
-learned proposal when enabled) + shadow rays + G-buffer write (k_trace) and the
-shading/edge pass (k_shade), with the mesh and BVH already resident in HBM.
+edges + matcap shading.  A step is one frame: raygen + BVH traversal + shadow rays + G-buffer write
+(k_trace, k_finish) and the shading/edge pass (k_shade), with the mesh and BVH
+already resident in HBM.  The learned-bounded frame adds the proposal network
+(k_infer, bf16 MFMA) before the primary trace, which then traverses with
+tmax = prediction * (1 + margin) and falls back to the unbounded traversal.
 
-value = (primary + shadow rays) / GPU time per frame, in Mrays/s.
+value = (primary + shadow rays) / wall time per frame of the learned-bounded
+frame (network included), in Mrays/s; plain_mrays is the same frame without
+the network.  Both frames' G-buffers must be identical (reported).
 
 The JSON line also carries:
   roofline      the trace kernel's algorithmic bytes per launch (node visits x
@@ -51,6 +55,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline time")
     ap.add_argument("--profile-only", action="store_true", help="no CPU baseline, short output (for rocprofv3)")
     ap.add_argument("--rows", type=int, default=0, help="render only a centred band of this many rows")
+    ap.add_argument("--no-learned", action="store_true", help="plain traversal only")
+    ap.add_argument("--train-rays", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--autotune", type=int, default=1, help="autotune stack depth / rays per item / margin")
     return ap.parse_args()
 
 
@@ -179,21 +187,61 @@ def main():
     assert st["stack_overflow"] == 0
     rays_per_frame = st["primary_rays"] + st["shadow_rays"]
 
-    for _ in range(a.warmup):
-        r.render(fr, tg, readback=False)
-    r.synchronize()
-    r.reset_timings()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        r.render(fr, tg, readback=False)
-    r.synchronize()
-    wall = time.perf_counter() - t0
-    tm = r.timings()
-    trace_ms = float(np.mean(tm["trace_ms"]))
-    shadow_ms = float(np.mean(tm["shadow_ms"]))
-    shade_ms = float(np.mean(tm["shade_ms"]))
-    ms_per_step = wall / a.steps * 1e3
-    value = rays_per_frame * a.steps / wall / 1e6
+    tune = None
+    if a.autotune:
+        tune = r.autotune(fr, tg, repeats=5)
+        log("autotune (plain): %s" % tune)
+
+    def timed(frame):
+        for _ in range(a.warmup):
+            r.render(frame, tg, readback=False)
+        r.synchronize()
+        r.reset_timings()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            r.render(frame, tg, readback=False)
+        r.synchronize()
+        wall = time.perf_counter() - t0
+        tm = r.timings()
+        return wall, {k: float(np.mean(v)) for k, v in tm.items()}
+
+    wall, tm = timed(fr)
+    trace_ms, shadow_ms, shade_ms = tm["trace_ms"], tm["shadow_ms"], tm["shade_ms"]
+    plain_ms = wall / a.steps * 1e3
+    plain_value = rays_per_frame * a.steps / wall / 1e6
+    ms_per_step, value = plain_ms, plain_value
+
+    learned = None
+    if not a.no_learned:
+        log("training the proposal network on %d rays x %d epochs ..." % (a.train_rays, a.epochs))
+        rep_t = r.train_proposal(slot, n_rays=a.train_rays, epochs=a.epochs, seed=7)
+        log("training: %s" % rep_t)
+        lfr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags | ffi.LEARNED, light_dir=light,
+                           rect=rect, margin=0.05)
+        ltune = None
+        if a.autotune:
+            ltune = r.autotune(lfr, tg, repeats=5)
+            log("autotune (learned): %s" % ltune)
+            lfr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags | ffi.LEARNED, light_dir=light,
+                               rect=rect, margin=ltune["margin"])
+        r.set_counting(True)
+        lg, lim = r.render(lfr, tg)
+        lst = r.stats()
+        r.set_counting(False)
+        mism = int(sum(np.count_nonzero(lg[k].view(np.uint32) != gpu_g[k].view(np.uint32)) if lg[k].dtype.itemsize == 4
+                       else np.count_nonzero(lg[k] != gpu_g[k]) for k in lg.dtype.names))
+        lwall, ltm = timed(lfr)
+        learned = {"mrays": rays_per_frame * a.steps / lwall / 1e6, "ms_per_step": lwall / a.steps * 1e3,
+                   "mlp_ms": ltm["mlp_ms"], "trace_ms": ltm["trace_ms"], "shadow_stage_ms": ltm["shadow_ms"],
+                   "margin": lfr.margin, "resolved": int(lst["learned_resolved"]),
+                   "fallback": int(lst["learned_fallback"]), "hit_pixels": int(lst["hits"]),
+                   "primary_node_visits": int(lst["node_visits"]), "plain_primary_node_visits": int(st["node_visits"]),
+                   "gbuffer_mismatches_vs_plain": mism, "rgba_equal_plain": bool(np.array_equal(lim, gpu_im)),
+                   "training": {k: rep_t[k] for k in ("steps", "loss_first", "loss_last", "train_ms", "label_ms",
+                                                        "hit_fraction", "val_hit_accuracy", "val_mean_abs_error",
+                                                        "val_resolved")},
+                   "autotune": ltune}
+        ms_per_step, value = learned["ms_per_step"], learned["mrays"]
 
     node_bytes = 96 if a.width == 8 else 64  # quantized wide node
     npx = w * h
@@ -212,12 +260,15 @@ def main():
            "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
            "roofline": roof,
-           "plain_mrays": value,
-           "primary_mrays": st["primary_rays"] * a.steps / wall / 1e6,
+           "plain_mrays": plain_value, "plain_ms_per_step": plain_ms,
+           "learned": learned,
+           "learned_over_plain": (learned["mrays"] / plain_value) if learned else None,
+           "autotune_plain": tune,
+           "primary_mrays": st["primary_rays"] / (ms_per_step * 1e-3) / 1e6,
            "rays_per_frame": int(rays_per_frame), "primary_rays": int(st["primary_rays"]),
            "shadow_rays": int(st["shadow_rays"]), "hit_pixels": int(st["hits"]),
-           "node_visits_per_ray": st["node_visits"] / max(1, rays_per_frame),
-           "tri_tests_per_ray": st["tri_tests"] / max(1, rays_per_frame),
+           "primary_node_visits_per_ray": st["node_visits"] / max(1, st["primary_rays"]),
+           "primary_tri_tests_per_ray": st["tri_tests"] / max(1, st["primary_rays"]),
            "trace_ms": trace_ms, "shadow_stage_ms": shadow_ms, "shade_ms": shade_ms,
            "shadow_node_visits_per_ray": st["shadow_node_visits"] / max(1, st["shadow_rays"]),
            "bvh": {"width": a.width, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),

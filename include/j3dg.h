@@ -48,8 +48,19 @@ extern "C" {
 #define J3DG_DIRECTIONAL_LIGHT (1u << 7)
 /* render_scene(nullptr) (j3d/canvas.cpp:883-890): only db_id and object_id are reset. */
 #define J3DG_NO_SCENE (1u << 8)
-/* Use the mesh's trained tmax proposal for primary rays (exact: falls back to plain traversal). */
+/* Learned bound for primary rays: the objects' proposal networks predict each
+   pixel's closest-hit distance; the trace first traverses with tmax = that
+   prediction * (1 + margin), which is exact because any hit inside the bound
+   is the global closest hit, and a ray that finds nothing there falls back to
+   the unbounded traversal.  No counterpart in the reference. */
 #define J3DG_LEARNED (1u << 9)
+/* With J3DG_LEARNED: skip the network and use the target's proposal buffer as
+   it stands (from an earlier learned frame or j3dg_write_target_proposal). */
+#define J3DG_KEEP_PROPOSAL (1u << 10)
+
+/* Parameters of the proposal network (78-64-64-64-2): W1[64][78], b1[64],
+   W2[64][64], b2, W3[64][64], b3, W4[2][64], b4[2]. */
+#define J3DG_MLP_PARAMS 13506
 
 /* Render stages */
 #define J3DG_STAGE_TRACE 1u /* canvas::render_scene: G-buffer */
@@ -105,6 +116,31 @@ typedef struct j3dg_mesh_info {
 } j3dg_mesh_info;
 
 /* Traversal tuning (chosen by the autotuner; 0 = default). */
+/* Training of a mesh's proposal network (j3dg_default_train_opts fills the defaults). */
+typedef struct j3dg_train_opts {
+  uint64_t n_rays;          /* training rays, labelled by the plain traversal (default 2^24) */
+  uint32_t epochs;          /* passes over the rays (default 4) */
+  uint32_t batch;           /* rays per Adam step, a multiple of 128 (default 16384) */
+  float lr, lr_final;       /* Adam learning rate, cosine-decayed from lr to lr_final */
+  uint64_t seed;
+  uint32_t rays_per_camera; /* rays sharing one random camera (default 1024) */
+  float dist_min, dist_max; /* camera distance from the box centre, in half-diagonals (1.2, 3.5) */
+  float fov_deg, aspect;    /* frustum of the random cameras (60 degrees vertical, 16:9) */
+  uint32_t n_validation;    /* held-out rays for the report (default 65536) */
+  uint32_t warm_start;      /* 1: start from the mesh's current network (resume), else a fresh init */
+} j3dg_train_opts;
+
+typedef struct j3dg_train_report {
+  float loss_first, loss_last; /* mean loss of the first / last 1% of the steps */
+  uint64_t steps;
+  float hit_fraction;          /* of the training + validation rays */
+  float label_ms, train_ms;    /* ray generation + labelling, optimisation */
+  float val_hit_accuracy;      /* held-out rays: hit/miss predicted correctly */
+  float val_mean_abs_error;    /* held-out hits: mean |y0 - s| (s: the label, in units of R) */
+  float val_resolved[4];       /* held-out hits a bounded pass resolves at margins 0, 0.01, 0.03, 0.1 */
+  float norm[4];               /* input normalisation: box centre, half-diagonal R */
+} j3dg_train_report;
+
 typedef struct j3dg_tuning {
   uint32_t stack_depth;    /* LDS stack entries per lane: 8, 16 or 32 */
   uint32_t rays_per_item;  /* rays each lane takes per queue fetch: 1, 2 or 4 */
@@ -187,6 +223,49 @@ int j3dg_read_target(j3dg_ctx* ctx, uint32_t target, j3dg_pixel* gbuf, uint32_t*
 /* Many frames (cameras x meshes of uneven size) in one persistent launch,
    captured in a HIP graph and replayed while the batch is unchanged. */
 int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n);
+
+/* ---- autotuner (no counterpart in the reference) -------------------------
+   Times the trace stages of `frame` (median of `repeats` frames, default 5)
+   over LDS stack depth {8,16,32} x rays per work item {1,2,4}, applies the
+   fastest tuning to the context and, for a J3DG_LEARNED frame, also picks the
+   fastest bound margin (returned; the caller puts it in its frames).  BVH
+   width {4,8} is the caller's choice between two uploads of the mesh. */
+typedef struct j3dg_autotune_report {
+  j3dg_tuning best;
+  float best_margin;
+  float best_ms;      /* trace stages of the frame with the chosen settings */
+  float baseline_ms;  /* with the tuning in effect before the call */
+  uint32_t n_tried;
+} j3dg_autotune_report;
+int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, uint32_t repeats,
+                  j3dg_autotune_report* report);
+
+/* ---- learned proposal (no counterpart in the reference) ---------------------
+   Trains the mesh's proposal network on the GPU: random-camera rays around the
+   mesh in its own coordinates, labelled by the plain traversal, fitted with
+   the bf16-MFMA forward/backward kernels and Adam.  The network then serves
+   every J3DG_LEARNED frame that contains the mesh (any placement). */
+int j3dg_default_train_opts(j3dg_train_opts* opts);
+int j3dg_train_proposal(j3dg_ctx* ctx, uint32_t slot, const j3dg_train_opts* opts, j3dg_train_report* report);
+/* Checkpoint: J3DG_MLP_PARAMS floats and the normalisation (centre xyz, R). */
+int j3dg_get_proposal(j3dg_ctx* ctx, uint32_t slot, float* params, float norm[4]);
+/* params == NULL removes the mesh's network. */
+int j3dg_set_proposal(j3dg_ctx* ctx, uint32_t slot, const float* params, const float norm[4]);
+/* The target's proposal buffer: w*h pairs (tmax bound, hit logit); a pixel with
+   logit <= 0 takes the plain traversal.  For J3DG_KEEP_PROPOSAL frames. */
+int j3dg_write_target_proposal(j3dg_ctx* ctx, uint32_t target, const float* pairs);
+int j3dg_read_target_proposal(j3dg_ctx* ctx, uint32_t target, float* pairs);
+/* The network's kernels on host arrays (rays: n x 8 floats, object space;
+   labels: n x 2 ((t |d| - a) / R, hit), a = (c - o) . d / |d|, see mlp_oracle.cpp).  forward -> out: n x 2 (y0, y1);
+   gradients -> grads: J3DG_MLP_PARAMS floats of the mean loss
+   hit * 10 (y0 - s)^2 + softplus(y1) - hit * y1, and *loss;
+   adam_step: one Adam update (beta 0.9 / 0.999, eps 1e-8) of params, m, v. */
+int j3dg_mlp_forward(j3dg_ctx* ctx, const float* params, const float norm[4], const float* rays, uint64_t n,
+                     float* out);
+int j3dg_mlp_gradients(j3dg_ctx* ctx, const float* params, const float norm[4], const float* rays,
+                       const float* labels, uint64_t n, float* grads, float* loss);
+int j3dg_mlp_adam_step(j3dg_ctx* ctx, float* params, float* m, float* v, const float* grads, uint32_t step,
+                       float lr);
 
 /* ---- raw ray queries (world space; used by tests and by the learned proposal) ----
    rays: n x 8 floats (ox,oy,oz,dx,dy,dz,t_near,t_far) in host memory.

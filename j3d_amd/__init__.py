@@ -8,4 +8,4 @@ binding used by the tests and the benchmark.
 from . import ffi  # noqa: F401
 from .api import (Frame, J3dgError, Renderer, bbox, camera_with_vertical_fov, default_camera,  # noqa: F401
                   icosphere, invert_orthonormal, make_db_id, make_frame, make_matcap, orbit_camera, projection,
-                  random_hemisphere, scene_framing, torus, translation_camera)
+                  random_hemisphere, read_proposal, scene_framing, torus, translation_camera, write_proposal)

@@ -6,6 +6,7 @@ canvas::render_scene and canvas::canvas_to_image as HIP kernels and, on
 request, reads the G-buffer (`pixel`, j3d/pixel.h) and RGBA image back.
 """
 import ctypes as C
+import os
 import math
 
 import numpy as np
@@ -195,6 +196,45 @@ def make_frame(width, height, camera, cs, cs_inv, pivot, diagonal, objects, **kw
 
 # --------------------------------------------------------------- renderer ----
 
+PROPOSAL_MAGIC = b"J3DGMLP\x00"
+PROPOSAL_VERSION = 1
+PROPOSAL_LAYERS = (78, 64, 64, 64, 2)
+
+
+def write_proposal(path, params, norm):
+    """Versioned binary checkpoint of a proposal network: 8-byte magic, uint32
+    version, uint32 layer count + sizes, 4 float32 normalisation values, then
+    the float32 parameters (W1, b1, ..., W4, b4), all little-endian."""
+    p = np.ascontiguousarray(params, "<f4")
+    nrm = np.ascontiguousarray(norm, "<f4")
+    assert p.size == ffi.MLP_PARAMS and nrm.size == 4
+    with open(path, "wb") as f:
+        f.write(PROPOSAL_MAGIC)
+        f.write(np.array([PROPOSAL_VERSION, len(PROPOSAL_LAYERS)] + list(PROPOSAL_LAYERS), "<u4").tobytes())
+        f.write(nrm.tobytes())
+        f.write(p.tobytes())
+
+
+def read_proposal(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:8] != PROPOSAL_MAGIC:
+        raise ValueError("not a proposal checkpoint: %s" % path)
+    head = np.frombuffer(data, "<u4", 2, 8)
+    if int(head[0]) != PROPOSAL_VERSION:
+        raise ValueError("unsupported proposal checkpoint version %d" % int(head[0]))
+    nl = int(head[1])
+    layers = tuple(int(x) for x in np.frombuffer(data, "<u4", nl, 16))
+    if layers != PROPOSAL_LAYERS:
+        raise ValueError("checkpoint network %s differs from %s" % (layers, PROPOSAL_LAYERS))
+    off = 16 + 4 * nl
+    nrm = np.frombuffer(data, "<f4", 4, off).copy()
+    p = np.frombuffer(data, "<f4", ffi.MLP_PARAMS, off + 16).copy()
+    if len(data) != off + 16 + 4 * ffi.MLP_PARAMS:
+        raise ValueError("truncated or oversized proposal checkpoint")
+    return p, nrm
+
+
 class Renderer:
     """One j3dg context on one GPU."""
 
@@ -360,6 +400,115 @@ class Renderer:
                                          ffi.ptr(out["tri"]), ffi.ptr(out["t"]), ffi.ptr(out["u"]),
                                          ffi.ptr(out["v"])), "j3dg_trace_rays")
         return out
+
+    def autotune(self, frame, target, repeats=5, cache=None, key=None):
+        """Autotunes stack depth x rays per item (and the margin of a learned
+        frame) on `frame` (j3dg_autotune) and applies the result.  With `cache`
+        (a JSON file path) results are stored and reused under `key`, e.g.
+        "<triangles>@<width>x<height>"."""
+        import json
+        if cache and key and os.path.exists(cache):
+            with open(cache) as f:
+                db = json.load(f)
+            if key in db:
+                e = db[key]
+                self.set_tuning(**e["tuning"])
+                return dict(e, cached=True)
+        rep = ffi.AutotuneReport()
+        d = frame.desc()
+        self._c(self.lib.j3dg_autotune(self.ctx, C.byref(d), int(target), int(repeats), C.byref(rep)), "j3dg_autotune")
+        out = {"tuning": {k: getattr(rep.best, k) for k, _ in ffi.Tuning._fields_}, "margin": rep.best_margin,
+               "best_ms": rep.best_ms, "baseline_ms": rep.baseline_ms, "n_tried": rep.n_tried, "cached": False}
+        if cache and key:
+            db = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    db = json.load(f)
+            db[key] = {k: v for k, v in out.items() if k != "cached"}
+            with open(cache, "w") as f:
+                json.dump(db, f, indent=1, sort_keys=True)
+        return out
+
+    # ---- learned proposal ------------------------------------------------
+    def train_proposal(self, slot, **opts):
+        """Trains the mesh's tmax proposal network on the GPU (j3dg_train_proposal);
+        keyword arguments override j3dg_default_train_opts fields.  Returns the report."""
+        o = ffi.TrainOpts()
+        self._c(self.lib.j3dg_default_train_opts(C.byref(o)), "j3dg_default_train_opts")
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        rep = ffi.TrainReport()
+        self._c(self.lib.j3dg_train_proposal(self.ctx, int(slot), C.byref(o), C.byref(rep)), "j3dg_train_proposal")
+        out = {k: getattr(rep, k) for k, _ in ffi.TrainReport._fields_}
+        out["val_resolved"] = list(rep.val_resolved)
+        out["norm"] = list(rep.norm)
+        return out
+
+    def get_proposal(self, slot):
+        p = np.zeros(ffi.MLP_PARAMS, np.float32)
+        nrm = np.zeros(4, np.float32)
+        self._c(self.lib.j3dg_get_proposal(self.ctx, int(slot), ffi.ptr(p), ffi.ptr(nrm)), "j3dg_get_proposal")
+        return p, nrm
+
+    def set_proposal(self, slot, params, norm):
+        if params is None:
+            self._c(self.lib.j3dg_set_proposal(self.ctx, int(slot), None, None), "j3dg_set_proposal")
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = np.ascontiguousarray(norm, np.float32)
+        assert p.size == ffi.MLP_PARAMS and nrm.size == 4
+        self._c(self.lib.j3dg_set_proposal(self.ctx, int(slot), ffi.ptr(p), ffi.ptr(nrm)), "j3dg_set_proposal")
+
+    def save_proposal(self, slot, path):
+        """Checkpoint: magic, version, layer sizes, normalisation, parameters (little-endian)."""
+        p, nrm = self.get_proposal(slot)
+        write_proposal(path, p, nrm)
+
+    def load_proposal(self, slot, path):
+        p, nrm = read_proposal(path)
+        self.set_proposal(slot, p, nrm)
+
+    def write_proposal(self, target, pairs):
+        w, h = self._targets[int(target)]
+        a = np.ascontiguousarray(pairs, np.float32).reshape(h, w, 2)
+        self._c(self.lib.j3dg_write_target_proposal(self.ctx, int(target), ffi.ptr(a)), "j3dg_write_target_proposal")
+
+    def read_proposal(self, target):
+        w, h = self._targets[int(target)]
+        a = np.zeros((h, w, 2), np.float32)
+        self._c(self.lib.j3dg_read_target_proposal(self.ctx, int(target), ffi.ptr(a)), "j3dg_read_target_proposal")
+        return a
+
+    def mlp_forward(self, params, norm, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 2), np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = np.ascontiguousarray(norm, np.float32)
+        self._c(self.lib.j3dg_mlp_forward(self.ctx, ffi.ptr(p), ffi.ptr(nrm), ffi.ptr(r), r.shape[0], ffi.ptr(out)),
+                "j3dg_mlp_forward")
+        return out
+
+    def mlp_gradients(self, params, norm, rays, labels):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        lab = np.ascontiguousarray(labels, np.float32).reshape(-1, 2)
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = np.ascontiguousarray(norm, np.float32)
+        g = np.zeros(ffi.MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._c(self.lib.j3dg_mlp_gradients(self.ctx, ffi.ptr(p), ffi.ptr(nrm), ffi.ptr(r), ffi.ptr(lab), r.shape[0],
+                                            ffi.ptr(g), C.byref(loss)), "j3dg_mlp_gradients")
+        return g, loss.value
+
+    def mlp_adam_step(self, params, m, v, grads, step, lr):
+        p = np.array(params, np.float32)
+        mm = np.array(m, np.float32)
+        vv = np.array(v, np.float32)
+        g = np.ascontiguousarray(grads, np.float32)
+        self._c(self.lib.j3dg_mlp_adam_step(self.ctx, ffi.ptr(p), ffi.ptr(mm), ffi.ptr(vv), ffi.ptr(g), int(step),
+                                            float(lr)), "j3dg_mlp_adam_step")
+        return p, mm, vv
 
     def any_hit(self, slots, rays):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -30,7 +30,7 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-flush-denormals-to
              "-Wno-unused-result", "-Wno-unused-value"]
 
 HIP_SOURCES = ["j3dg.hip", "mlp.hip", "batch.hip"]
-CPP_SOURCES = ["bvh_build.cpp", "scenegen.cpp", "host_tables.cpp"]
+CPP_SOURCES = ["bvh_build.cpp", "scenegen.cpp", "host_tables.cpp", "autotune.cpp"]
 
 
 def _run(cmd):

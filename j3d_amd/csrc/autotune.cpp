@@ -1,0 +1,81 @@
+// autotune.cpp -- the traversal autotuner, a client of the public C ABI.
+//
+// For one representative frame it times the trace stages (primary trace,
+// proposal network, finish + shadow trace; HIP events, median of `repeats`
+// frames after one warm-up) over LDS stack depth {8, 16, 32} x rays per
+// work item {1, 2, 4}, keeps the fastest, and for J3DG_LEARNED frames then
+// sweeps the bound margin.  BVH width is chosen by the caller, which owns the
+// mesh uploads (one slot per width); the result cache lives with the caller.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "j3dg.h"
+
+namespace {
+
+int time_frame(j3dg_ctx* ctx, const j3dg_frame_desc& fd, uint32_t target, uint32_t repeats, float* ms) {
+  int rc = j3dg_render(ctx, &fd, target, J3DG_STAGE_TRACE, nullptr, nullptr);
+  if (rc) return rc;
+  if ((rc = j3dg_synchronize(ctx)) || (rc = j3dg_reset_timings(ctx))) return rc;
+  for (uint32_t i = 0; i < repeats; ++i)
+    if ((rc = j3dg_render(ctx, &fd, target, J3DG_STAGE_TRACE, nullptr, nullptr))) return rc;
+  std::vector<float> tr(repeats), ml(repeats), sh(repeats), tot(repeats);
+  uint32_t n = 0;
+  if ((rc = j3dg_get_timings(ctx, tr.data(), nullptr, ml.data(), sh.data(), repeats, &n))) return rc;
+  if (n == 0) return J3DG_ERR_INTERNAL;
+  for (uint32_t i = 0; i < n; ++i) tot[i] = tr[i] + ml[i] + sh[i];
+  std::nth_element(tot.begin(), tot.begin() + n / 2, tot.begin() + n);
+  *ms = tot[n / 2];
+  return J3DG_OK;
+}
+
+}  // namespace
+
+extern "C" int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, uint32_t repeats,
+                             j3dg_autotune_report* rep) {
+  if (!ctx || !frame || !rep) return J3DG_ERR_INVALID_ARG;
+  if (repeats == 0) repeats = 5;
+  if (repeats > 256) return J3DG_ERR_INVALID_ARG;
+  std::memset(rep, 0, sizeof(*rep));
+  j3dg_tuning base;
+  int rc = j3dg_get_tuning(ctx, &base);
+  if (rc) return rc;
+  rc = time_frame(ctx, *frame, target, repeats, &rep->baseline_ms);
+  if (rc) return rc;
+  j3dg_tuning best = base;
+  float best_ms = rep->baseline_ms;
+  const uint32_t depths[3] = {8, 16, 32}, rpis[3] = {1, 2, 4};
+  for (uint32_t d : depths)
+    for (uint32_t r : rpis) {
+      j3dg_tuning t = base;
+      t.stack_depth = d;
+      t.rays_per_item = r;
+      if ((rc = j3dg_set_tuning(ctx, &t))) return rc;
+      float ms = 0.f;
+      if ((rc = time_frame(ctx, *frame, target, repeats, &ms))) {
+        j3dg_set_tuning(ctx, &base);
+        return rc;
+      }
+      rep->n_tried++;
+      if (ms < best_ms) best_ms = ms, best = t;
+    }
+  if ((rc = j3dg_set_tuning(ctx, &best))) return rc;
+  rep->best = best;
+  rep->best_ms = best_ms;
+  rep->best_margin = frame->margin;
+  if (frame->flags & J3DG_LEARNED) {
+    const float margins[7] = {0.f, 0.005f, 0.01f, 0.02f, 0.05f, 0.1f, 0.2f};
+    float bm = best_ms;
+    for (float m : margins) {
+      j3dg_frame_desc fd = *frame;
+      fd.margin = m;
+      float ms = 0.f;
+      if ((rc = time_frame(ctx, fd, target, repeats, &ms))) return rc;
+      rep->n_tried++;
+      if (ms < bm) bm = ms, rep->best_margin = m;
+    }
+    rep->best_ms = bm;
+  }
+  return J3DG_OK;
+}

@@ -29,16 +29,6 @@ using namespace j3d;
 
 namespace j3d {
 
-// Primary rays come in tiles of 8x8 pixels: ray id = tile * 64 + lane.
-__device__ inline bool tile_pixel(const FrameParams& f, uint32_t tile, uint32_t l, uint32_t& px, uint32_t& py) {
-  uint32_t tx = tile % f.tiles_x, ty = tile / f.tiles_x;
-  px = (uint32_t)f.x0 + tx * 8 + (l & 7);
-  py = (uint32_t)f.y0 + ty * 8 + (l >> 3);
-  return px <= (uint32_t)f.x1 && py <= (uint32_t)f.y1;
-}
-
-enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3 };
-
 // Event bits a lane raises during one loop trip; the wave turns them into
 // uniform counts with ballot + popcount (no per-lane counters).
 enum : uint32_t { kEvPrimary = 1u, kEvHit = 2u, kEvShadowed = 4u, kEvResolved = 8u, kEvFallback = 16u };
@@ -111,21 +101,15 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         if (MODE == kModeFrame) {
           uint32_t px, py;
           if (!tile_pixel(f, id >> 6, id & 63u, px, py)) continue;
-          // j3d/canvas.cpp:773-782
-          const float Wf = (float)f.w, Hf = (float)f.h;
-          f4 sp = {2.f * (((float)px + 0.5f) / Wf) - 1.f, 2.f * (((float)py + 0.5f) / Hf) - 1.f, f.near_plane, 1.f};
-          f4 dir = mat_vec(f.proj_inv, sp);
-          dir.w = 0.f;
-          wd = mat_vec(f.cs, dir);
-          wo = mat_vec(f.cs, f4{0.f, 0.f, 0.f, 1.f});
+          primary_ray(f, px, py, wo, wd);
           tn = f.diagonal / 100.f;
           tf = FLT_MAX;
           pix = py * f.w + px;
           phase = 2;
           if (learned) {
-            float2 pr = f.proposal[pix];
-            if (pr.y > 0.f) {  // predicted hit: bounded pass first
-              tf = pr.x * (1.f + f.margin);
+            const float2 pr = f.proposal[pix];
+            if (pr.y > 0.f) {  // predicted hit: bounded pass first (pr.x = t_pred * (1 + margin))
+              tf = pr.x;
               phase = 1;
             }
           }
@@ -223,11 +207,15 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           }
           phase = 0;
         } else if (phase == 1 && !best.found) {
-          // the bounded pass found nothing below t_pred * (1 + margin): exact fallback
+          // The bounded pass tested every triangle with t in [t_near, bound] and
+          // found none, so the closest hit lies beyond the bound: the fallback
+          // searches (bound, inf) only.  A bound that is NaN or below t_near
+          // proves nothing and the fallback starts at t_near.
           ev |= kEvFallback;
           phase = 2;
           st.sp = 0;
-          begin_object(s, objs, 0, wo, wd, s.r.t_near, FLT_MAX, best);
+          const float t0 = f.diagonal / 100.f, tb = s.r.t_far;
+          begin_object(s, objs, 0, wo, wd, tb >= t0 ? tb : t0, FLT_MAX, best);
         } else {
           if (phase == 1) ev |= kEvResolved;
           phase = 0;
@@ -479,16 +467,16 @@ __global__ void k_no_scene(j3dg_pixel* g, uint64_t n) {
   g[i].object_id = kMiss;
 }
 
+// A fresh proposal buffer predicts no hit anywhere (plain traversal).
+__global__ void k_clear_proposal(float2* p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float2(0.f, -1.f);
+}
+
 
 // ============================================================= host side ===
 
 namespace {
-
-#define HIP_TRY(expr, code)                                                         \
-  do {                                                                              \
-    hipError_t e_ = (expr);                                                         \
-    if (e_ != hipSuccess) return ctx->fail(code, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 template <int W, int D, int MODE>
 void* trace_kernel_wd(bool count) {
@@ -554,10 +542,16 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
     std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
     o.db_id = m.db_id;
     o.width = m.info.width;
+    o.mlp = m.has_proposal ? m.d_mlp_packed : nullptr;
+    std::memcpy(o.mlp_norm, m.mlp_norm, sizeof(o.mlp_norm));
     max_stack = std::max(max_stack, m.max_stack);
   }
-  if (n && tab.size() * sizeof(DevObject) == obj_cache.size() &&
-      std::memcmp(obj_cache.data(), tab.data(), obj_cache.size()) == 0)
+  return upload_object_table(tab.data(), n);
+}
+
+int Ctx::upload_object_table(const DevObject* tab, uint32_t n) {
+  if (n && n * sizeof(DevObject) == obj_cache.size() &&
+      std::memcmp(obj_cache.data(), tab, obj_cache.size()) == 0)
     return J3DG_OK;
   if (n > obj_capacity) {
     if (d_objs) hipFree(d_objs);
@@ -570,10 +564,9 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
   if (n) {
     Ctx* ctx = this;
     HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
-    HIP_TRY(hipMemcpy(d_objs, tab.data(), n * sizeof(DevObject), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipMemcpy(d_objs, tab, n * sizeof(DevObject), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
   }
-  obj_cache.assign(reinterpret_cast<const uint8_t*>(tab.data()),
-                   reinterpret_cast<const uint8_t*>(tab.data()) + n * sizeof(DevObject));
+  obj_cache.assign(reinterpret_cast<const uint8_t*>(tab), reinterpret_cast<const uint8_t*>(tab) + n * sizeof(DevObject));
   return J3DG_OK;
 }
 
@@ -678,6 +671,13 @@ int Ctx::launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool
   return J3DG_OK;
 }
 
+int Ctx::reset_counters() {
+  Ctx* ctx = this;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, d_queue, d_counters, 32u);
+  HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
 int Ctx::launch_shade(const ShadeParams& sp) {
   Ctx* ctx = this;
   dim3 grid((sp.w + 63) / 64, (sp.h + 3) / 4);
@@ -697,15 +697,6 @@ using j3d::Ctx;
 using j3d::Mesh;
 using j3d::Target;
 
-#define CHECK_CTX()                         \
-  if (!ctx) return J3DG_ERR_INVALID_ARG;    \
-  std::lock_guard<std::mutex> lock_(ctx->mu)
-
-#define HIP_CALL(expr, code)                                                        \
-  do {                                                                              \
-    hipError_t e_ = (expr);                                                         \
-    if (e_ != hipSuccess) return ctx->fail(code, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 extern "C" {
 
@@ -1110,6 +1101,43 @@ int j3dg_read_target(j3dg_ctx* h, uint32_t target, j3dg_pixel* gbuf, uint32_t* r
   return J3DG_OK;
 }
 
+static int ensure_proposal(Ctx* ctx, Target& tg) {
+  if (tg.proposal) return J3DG_OK;
+  HIP_CALL(hipMalloc(&tg.proposal, (size_t)tg.w * tg.h * sizeof(float2)), J3DG_ERR_OUT_OF_MEMORY);
+  hipLaunchKernelGGL(k_clear_proposal, dim3((unsigned)(((size_t)tg.w * tg.h + 255) / 256)), dim3(256), 0, ctx->stream,
+                     tg.proposal, (uint64_t)tg.w * tg.h);
+  HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
+int j3dg_write_target_proposal(j3dg_ctx* h, uint32_t target, const float* pairs) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (!pairs) return ctx->fail(J3DG_ERR_INVALID_ARG, "null proposal");
+  if (target >= ctx->targets.size() || !ctx->targets[target].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target");
+  Target& tg = ctx->targets[target];
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  int rc = ensure_proposal(ctx, tg);
+  if (rc) return rc;
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipMemcpy(tg.proposal, pairs, (size_t)tg.w * tg.h * sizeof(float2), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
+int j3dg_read_target_proposal(j3dg_ctx* h, uint32_t target, float* pairs) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (!pairs) return ctx->fail(J3DG_ERR_INVALID_ARG, "null proposal");
+  if (target >= ctx->targets.size() || !ctx->targets[target].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target");
+  Target& tg = ctx->targets[target];
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  int rc = ensure_proposal(ctx, tg);
+  if (rc) return rc;
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipMemcpy(pairs, tg.proposal, (size_t)tg.w * tg.h * sizeof(float2), hipMemcpyDeviceToHost), J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
 int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_t stages, j3dg_pixel* gbuf_out,
                 uint32_t* rgba_out) {
   Ctx* ctx = reinterpret_cast<Ctx*>(h);
@@ -1143,7 +1171,21 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
         rc = ctx->build_object_table(fd->objects, fd->n_objects, max_stack, width);
         if (rc) return rc;
       }
-      rc = ctx->launch_trace(fp, std::max<uint32_t>(max_stack, 1), width, (fd->flags & J3DG_LEARNED) != 0);
+      if ((fd->flags & J3DG_LEARNED) && fp.n_obj) {
+        bool any_net = false;
+        for (uint32_t i = 0; i < fd->n_objects; ++i) any_net = any_net || ctx->meshes[fd->objects[i]].has_proposal;
+        const bool keep = (fd->flags & J3DG_KEEP_PROPOSAL) != 0;
+        if (keep || any_net) {
+          rc = ensure_proposal(ctx, tg);
+          if (rc) return rc;
+          fp.proposal = tg.proposal;
+          if (!keep) {
+            rc = ctx->launch_mlp_frame(fp, fp.n_obj);
+            if (rc) return rc;
+          }
+        }
+      }
+      rc = ctx->launch_trace(fp, std::max<uint32_t>(max_stack, 1), width, fp.proposal != nullptr);
       if (rc) return rc;
       ctx->traced = true;
     }

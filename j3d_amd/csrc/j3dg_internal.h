@@ -25,6 +25,8 @@ struct DevObject {
   uint32_t db_id;
   uint32_t width;
   uint32_t pad[2];
+  const void* mlp;         // learned proposal: packed network (nullptr = none)
+  float mlp_norm[4];       // its input normalisation: box centre (object space), half-diagonal
 };
 
 // Frame parameters of the trace kernel (passed by value).
@@ -41,9 +43,11 @@ struct FrameParams {
   uint32_t flags;
   uint32_t n_obj;
   j3dg_pixel* gbuf;
-  const float2* proposal;   // learned: per-pixel (t_pred, hit logit), rect-indexed
+  const float2* proposal;   // learned: per-pixel (t bound, hit logit), indexed y * w + x
   float margin;
 };
+
+enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3 };
 
 // Ray-buffer queries (j3dg_trace_rays / j3dg_any_hit_rays).
 struct RayQuery {
@@ -89,8 +93,10 @@ struct Mesh {
   uint32_t* d_idx = nullptr;
   float* d_normals = nullptr;
   std::vector<float> normals_host;
-  // learned proposal (weights on the device, see mlp.hip)
-  void* d_mlp = nullptr;
+  // learned proposal (see mlp.hip): fp32 parameters, packed bf16 fragments
+  float* d_mlp = nullptr;
+  void* d_mlp_packed = nullptr;
+  float mlp_norm[4] = {0.f, 0.f, 0.f, 1.f};
   bool has_proposal = false;
   void release() {
     if (d_nodes) hipFree(d_nodes);
@@ -99,7 +105,9 @@ struct Mesh {
     if (d_idx) hipFree(d_idx);
     if (d_normals) hipFree(d_normals);
     if (d_mlp) hipFree(d_mlp);
+    if (d_mlp_packed) hipFree(d_mlp_packed);
     d_nodes = nullptr, d_tris = nullptr, d_verts = nullptr, d_idx = nullptr, d_normals = nullptr, d_mlp = nullptr;
+    d_mlp_packed = nullptr;
     normals_host.clear();
     has_proposal = false;
     live = false;
@@ -111,10 +119,12 @@ struct Target {
   uint32_t w = 0, h = 0;
   j3dg_pixel* gbuf = nullptr;
   uint32_t* rgba = nullptr;
+  float2* proposal = nullptr;  // learned: per-pixel (t bound, hit logit), allocated on first use
   void release() {
     if (gbuf) hipFree(gbuf);
     if (rgba) hipFree(rgba);
-    gbuf = nullptr, rgba = nullptr;
+    if (proposal) hipFree(proposal);
+    gbuf = nullptr, rgba = nullptr, proposal = nullptr;
     live = false;
   }
 };
@@ -154,6 +164,10 @@ struct Ctx {
   int fail(int code, const std::string& msg);
   int ensure_spill(uint64_t lanes, uint32_t entries);
   int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
+  int upload_object_table(const DevObject* tab, uint32_t n);
+  // learned proposal for the frame's objects into fp.proposal (mlp.hip)
+  int launch_mlp_frame(const FrameParams& fp, uint32_t n_obj);
+  int reset_counters();  // work-queue heads and statistics counters
   int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned);
   int launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
                           uint32_t max_stack, int width, uint32_t* qcounter);
@@ -230,3 +244,16 @@ inline void make_shade_params(const j3dg_frame_desc& fd, const Target& tg, const
 }
 
 }  // namespace j3d
+
+// Error plumbing shared by the library's translation units: record the HIP
+// error text in the context and return a j3dg status code.
+#define HIP_TRY(expr, code)                                                                         \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess) return ctx->fail(code, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define HIP_CALL(expr, code) HIP_TRY(expr, code)
+// Entry-point prologue: a valid context, held for the whole call.
+#define CHECK_CTX()                      \
+  if (!ctx) return J3DG_ERR_INVALID_ARG; \
+  std::lock_guard<std::mutex> lock_(ctx->mu)

@@ -56,6 +56,25 @@ __device__ inline float safe_rcp(float d) {
   return 1.f / a;
 }
 
+// Primary rays come in tiles of 8x8 pixels: ray id = tile * 64 + lane.
+__device__ inline bool tile_pixel(const FrameParams& f, uint32_t tile, uint32_t l, uint32_t& px, uint32_t& py) {
+  uint32_t tx = tile % f.tiles_x, ty = tile / f.tiles_x;
+  px = (uint32_t)f.x0 + tx * 8 + (l & 7);
+  py = (uint32_t)f.y0 + ty * 8 + (l >> 3);
+  return px <= (uint32_t)f.x1 && py <= (uint32_t)f.y1;
+}
+
+// World-space primary ray of pixel (px, py) (j3d/canvas.cpp:773-782): the
+// screen point through P^-1 (camera direction with z = -1), then CS.
+__device__ inline void primary_ray(const FrameParams& f, uint32_t px, uint32_t py, f4& wo, f4& wd) {
+  const float Wf = (float)f.w, Hf = (float)f.h;
+  f4 sp = {2.f * (((float)px + 0.5f) / Wf) - 1.f, 2.f * (((float)py + 0.5f) / Hf) - 1.f, f.near_plane, 1.f};
+  f4 dir = mat_vec(f.proj_inv, sp);
+  dir.w = 0.f;
+  wd = mat_vec(f.cs, dir);
+  wo = mat_vec(f.cs, f4{0.f, 0.f, 0.f, 1.f});
+}
+
 __device__ inline Ray make_object_ray(const float* inv_cs, f4 o, f4 d, float tn, float tf) {
   f4 oo = mat_vec(inv_cs, o);
   f4 dd = mat_vec(inv_cs, d);

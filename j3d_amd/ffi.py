@@ -25,6 +25,8 @@ VERTEXCOLORS = 1 << 6
 DIRECTIONAL_LIGHT = 1 << 7
 NO_SCENE = 1 << 8
 LEARNED = 1 << 9
+KEEP_PROPOSAL = 1 << 10
+MLP_PARAMS = 13506
 
 STAGE_TRACE = 1
 STAGE_SHADE = 2
@@ -80,6 +82,26 @@ class Stats(C.Structure):
                 ("trace_ms", C.c_float), ("shade_ms", C.c_float), ("mlp_ms", C.c_float), ("shadow_ms", C.c_float)]
 
 
+class AutotuneReport(C.Structure):
+    _fields_ = [("best", Tuning), ("best_margin", C.c_float), ("best_ms", C.c_float), ("baseline_ms", C.c_float),
+                ("n_tried", C.c_uint32)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("n_rays", C.c_uint64), ("epochs", C.c_uint32), ("batch", C.c_uint32),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("seed", C.c_uint64),
+                ("rays_per_camera", C.c_uint32), ("dist_min", C.c_float), ("dist_max", C.c_float),
+                ("fov_deg", C.c_float), ("aspect", C.c_float), ("n_validation", C.c_uint32),
+                ("warm_start", C.c_uint32)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("loss_first", C.c_float), ("loss_last", C.c_float), ("steps", C.c_uint64),
+                ("hit_fraction", C.c_float), ("label_ms", C.c_float), ("train_ms", C.c_float),
+                ("val_hit_accuracy", C.c_float), ("val_mean_abs_error", C.c_float),
+                ("val_resolved", C.c_float * 4), ("norm", C.c_float * 4)]
+
+
 class Camera(C.Structure):
     _fields_ = [("focal_length", C.c_float), ("film_w_inch", C.c_float), ("film_h_inch", C.c_float),
                 ("near_plane", C.c_float), ("far_plane", C.c_float), ("zoom", C.c_float),
@@ -124,6 +146,16 @@ SIGNATURES = {
     "j3dg_any_hit_rays": (_I, [_P, _P, _U32, _P, _U64, _P]),
     "j3dg_calibration_copy": (_I, [_P, _U64, _U32, C.POINTER(_F)]),
     "j3dg_debug_timeline": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
+    "j3dg_autotune": (_I, [_P, C.POINTER(FrameDesc), _U32, _U32, C.POINTER(AutotuneReport)]),
+    "j3dg_default_train_opts": (_I, [C.POINTER(TrainOpts)]),
+    "j3dg_train_proposal": (_I, [_P, _U32, C.POINTER(TrainOpts), C.POINTER(TrainReport)]),
+    "j3dg_get_proposal": (_I, [_P, _U32, _P, _P]),
+    "j3dg_set_proposal": (_I, [_P, _U32, _P, _P]),
+    "j3dg_write_target_proposal": (_I, [_P, _U32, _P]),
+    "j3dg_read_target_proposal": (_I, [_P, _U32, _P]),
+    "j3dg_mlp_forward": (_I, [_P, _P, _P, _P, _U64, _P]),
+    "j3dg_mlp_gradients": (_I, [_P, _P, _P, _P, _P, _U64, _P, C.POINTER(_F)]),
+    "j3dg_mlp_adam_step": (_I, [_P, _P, _P, _P, _P, _U32, _F]),
     # j3dg_host.h
     "j3dg_default_camera": (None, [C.POINTER(Camera)]),
     "j3dg_frustum": (None, [_F, _F, _F, _F, _F, _F, _P]),

@@ -84,6 +84,11 @@ def load():
         "orc_closest_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P, _P, _P, _P, _P, _P]),
         "orc_any_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P]),
         "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+        "orc_mlp_param_count": (C.c_int, []),
+        "orc_mlp_line": (None, [_P, C.c_uint64, _P, _P]),
+        "orc_mlp_features": (None, [_P, C.c_uint64, _P, _P]),
+        "orc_mlp_forward": (None, [_P, _P, _P, C.c_uint64, C.c_int, _P]),
+        "orc_mlp_gradients": (C.c_float, [_P, _P, _P, _P, C.c_uint64, C.c_int, _P]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -255,3 +260,43 @@ def intersect_triangle(o, d, v0, v1, v2, t_near=0.0, t_far=3.4028234663852886e38
     hit = load().orc_intersect_triangle(*[_p(x) for x in a], float(t_near), float(t_far), C.byref(t), C.byref(u),
                                         C.byref(v))
     return bool(hit), t.value, u.value, v.value
+
+
+# ---- learned proposal network (mlp_oracle.cpp) ----------------------------
+
+def mlp_param_count():
+    return int(load().orc_mlp_param_count())
+
+
+def mlp_line(rays, norm):
+    """Per ray: (|d|, a, x0..x5), the network's line input (mlp_oracle.cpp)."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    out = np.zeros((r.shape[0], 8), np.float32)
+    load().orc_mlp_line(_p(r), r.shape[0], _p(np.ascontiguousarray(norm, np.float32)), _p(out))
+    return out
+
+
+def mlp_features(rays, norm):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    out = np.zeros((r.shape[0], 78), np.float32)
+    load().orc_mlp_features(_p(r), r.shape[0], _p(np.ascontiguousarray(norm, np.float32)), _p(out))
+    return out
+
+
+def mlp_forward(params, norm, rays, bf16=True):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    out = np.zeros((r.shape[0], 2), np.float32)
+    load().orc_mlp_forward(_p(np.ascontiguousarray(params, np.float32)), _p(np.ascontiguousarray(norm, np.float32)),
+                           _p(r), r.shape[0], int(bool(bf16)), _p(out))
+    return out
+
+
+def mlp_gradients(params, norm, rays, labels, bf16=True):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    lab = np.ascontiguousarray(labels, np.float32).reshape(-1, 2)
+    g = np.zeros(mlp_param_count(), np.float32)
+    loss = load().orc_mlp_gradients(_p(np.ascontiguousarray(params, np.float32)),
+                                    _p(np.ascontiguousarray(norm, np.float32)), _p(r), _p(lab), r.shape[0],
+                                    int(bool(bf16)), _p(g))
+    return g, float(loss)
+
