@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--train-rays", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--autotune", type=int, default=1, help="autotune stack depth / rays per item / margin")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with a 1 GiB device copy (read + write), the reference for profiler byte counters")
     return ap.parse_args()
 
 
@@ -92,7 +94,7 @@ def workload(name):
 
 def read_traffic(workload_name):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (committed summary)."""
-    p = os.path.join(ROOT, "profiles", "trace_%s_traffic.json" % workload_name)
+    p = os.path.join(ROOT, "profiles", "k_trace_primary_%s_1080p_traffic.json" % workload_name)
     if not os.path.exists(p):
         return None
     try:
@@ -280,6 +282,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(v, t, fr, mc, cav, gpu_g, gpu_im, a.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    if a.calibrate:
+        out["calibration_copy_ms"] = r.calibration_copy(1 << 30, 2)
     r.close()
     print(json.dumps(out), flush=True)
 

@@ -716,8 +716,15 @@ __global__ void k_train_labels(const float* __restrict__ rays, const uint32_t* _
     labels[2 * (size_t)i] = hit ? (t[i] * dl - a) / norm4.w : 0.f;
     labels[2 * (size_t)i + 1] = hit ? 1.f : 0.f;
   }
-  const uint32_t c = (uint32_t)__popcll(__ballot(hit));
-  if ((threadIdx.x & 63u) == 0 && c) atomicAdd(n_hits, c);
+  // one atomic per block (per-wave atomics on one address serialise)
+  __shared__ uint32_t wave_hits[4];  // launched with 256 threads
+  const uint64_t b = __ballot(hit);
+  if ((threadIdx.x & 63u) == 0) wave_hits[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t c = wave_hits[0] + wave_hits[1] + wave_hits[2] + wave_hits[3];
+    if (c) atomicAdd(n_hits, c);
+  }
 }
 
 }  // namespace mlp

@@ -1,0 +1,84 @@
+"""The C ABI (include/*.h) on the CPU: every declared function is exported by
+the built libraries and bound by the ctypes layer, the record layouts the
+ctypes layer uses match the C compiler's, and status codes / version are as
+declared.  No GPU calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from j3d_amd import ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "j3d_amd", "lib", "libj3dg.so")
+
+
+def declared_functions(header):
+    text = open(os.path.join(INC, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(j3dg_[a-z0-9_]+)\s*\(", text)) - set(re.findall(r"#define\s+(j3dg_\w+)", text)))
+
+
+def exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], stdout=subprocess.PIPE, text=True, check=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+@pytest.mark.parametrize("header", ["j3dg.h", "j3dg_host.h", "j3dg_scenegen.h"])
+def test_every_declared_function_is_exported(header):
+    names = declared_functions(header)
+    assert names, header
+    exp = exported(LIB)
+    missing = [n for n in names if n not in exp]
+    assert not missing, missing
+
+
+def test_ctypes_binds_every_function():
+    lib = ffi.load()
+    names = set()
+    for h in ("j3dg.h", "j3dg_host.h", "j3dg_scenegen.h"):
+        names |= set(declared_functions(h))
+    for n in names:
+        assert hasattr(lib, n), n
+    unbound = sorted(n for n in names if n not in ffi.SIGNATURES)
+    assert not unbound, unbound
+
+
+def test_record_layouts_match_the_c_compiler(tmp_path):
+    structs = {"j3dg_pixel": None, "j3dg_frame_desc": ffi.FrameDesc, "j3dg_build_opts": ffi.BuildOpts,
+               "j3dg_mesh_info": ffi.MeshInfo, "j3dg_tuning": ffi.Tuning, "j3dg_stats": ffi.Stats,
+               "j3dg_train_opts": ffi.TrainOpts, "j3dg_train_report": ffi.TrainReport,
+               "j3dg_autotune_report": ffi.AutotuneReport, "j3dg_camera": ffi.Camera}
+    src = tmp_path / "sizes.c"
+    body = "\n".join('printf("%s %%zu\\n", sizeof(%s));' % (s, s) for s in structs)
+    src.write_text('#include <stdio.h>\n#include "j3dg.h"\n#include "j3dg_host.h"\nint main(void){%s return 0;}\n' % body)
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", "-I" + INC, "-o", str(exe), str(src)], check=True)
+    sizes = dict(l.split() for l in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,
+                                                   check=True).stdout.splitlines())
+    assert int(sizes["j3dg_pixel"]) == ffi.PIXEL_DTYPE.itemsize == 32
+    for name, cls in structs.items():
+        if cls is not None:
+            assert int(sizes[name]) == C.sizeof(cls), name
+
+
+def test_version_and_status_codes():
+    lib = ffi.load()
+    assert lib.j3dg_abi_version() == 1
+    text = open(os.path.join(INC, "j3dg.h")).read()
+    for code, name in ffi.ERRORS.items():
+        assert re.search(r"#define J3DG_ERR_\w+ %d\b" % code, text), (code, name)
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    lib = ffi.load()
+    assert lib.j3dg_create(0, None) == 1          # invalid argument
+    assert lib.j3dg_device_count(None) == 1
+    assert lib.j3dg_set_tuning(None, None) == 1   # no context
+    assert lib.j3dg_default_train_opts(None) == 1
+    o = ffi.TrainOpts()
+    assert lib.j3dg_default_train_opts(C.byref(o)) == 0
+    assert o.n_rays == 1 << 24 and o.batch % 128 == 0
