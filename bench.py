@@ -92,9 +92,10 @@ def workload(name):
     return v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc
 
 
-def read_traffic(workload_name):
+def read_traffic(workload_name, fused):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (committed summary)."""
-    p = os.path.join(ROOT, "profiles", "k_trace_primary_%s_1080p_traffic.json" % workload_name)
+    p = os.path.join(ROOT, "profiles", "k_trace_%s_%s_1080p_traffic.json" % ("fused" if fused else "primary",
+                                                                         workload_name))
     if not os.path.exists(p):
         return None
     try:
@@ -247,16 +248,28 @@ def main():
 
     node_bytes = 96 if a.width == 8 else 64  # quantized wide node
     npx = w * h
-    # primary k_trace launch: node and triangle reads + 20 B of G-buffer writes per pixel
-    algo_bytes = st["node_visits"] * node_bytes + st["tri_tests"] * 48 + 20 * npx
+    fused = r.get_tuning()["split_passes"] == 0
+    if fused:
+        # one k_trace launch: primary and shadow node / triangle reads, 32 B G-buffer
+        # write per pixel, and per hit 60 B of hit completion reads (indices,
+        # vertices, normal)
+        algo_bytes = ((st["node_visits"] + st["shadow_node_visits"]) * node_bytes +
+                      (st["tri_tests"] + st["shadow_tri_tests"]) * 48 + 32 * npx + 60 * st["hits"])
+        per_unit = "%d B per node visit, 48 B per triangle test (primary and shadow), 32 B per pixel, " \
+                   "60 B per hit" % node_bytes
+        kname = "k_trace (one launch: primary, hit completion, shadow)"
+    else:
+        # primary k_trace launch: node and triangle reads + 20 B of G-buffer writes per pixel
+        algo_bytes = st["node_visits"] * node_bytes + st["tri_tests"] * 48 + 20 * npx
+        per_unit = "%d B per node visit, 48 B per triangle test, 20 B per pixel" % node_bytes
+        kname = "k_trace (primary pass)"
     achieved = algo_bytes / (trace_ms * 1e-3)
-    traffic = read_traffic(a.workload)
+    traffic = read_traffic(a.workload, fused)
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": (traffic or {}).get("hbm_bytes_per_launch") if traffic else None,
-            "kernel": "k_trace", "launch_ms": trace_ms,
-            "algorithmic_bytes_per_launch": int(algo_bytes),
-            "per_unit": "%d B per node visit, 48 B per triangle test, 20 B per pixel" % node_bytes}
+            "kernel": kname, "launch_ms": trace_ms,
+            "algorithmic_bytes_per_launch": int(algo_bytes), "per_unit": per_unit}
 
     out = {"metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",

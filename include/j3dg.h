@@ -147,6 +147,8 @@ typedef struct j3dg_tuning {
   uint32_t coop_tail;      /* 1 = once a wave's rays run out, its last <= 8 rays get 8 lanes each */
   uint32_t block_size;     /* threads per workgroup: 64, 128 or 256 */
   uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves) */
+  uint32_t split_passes;   /* 0 = one launch per frame (a lane traces its pixel's shadow ray right after
+                              the primary hit); 1 = primary pass, hit completion, shadow pass */
 } j3dg_tuning;
 
 /* Statistics of the last j3dg_render / j3dg_render_batch. */

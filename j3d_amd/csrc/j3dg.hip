@@ -31,7 +31,53 @@ namespace j3d {
 
 // Event bits a lane raises during one loop trip; the wave turns them into
 // uniform counts with ballot + popcount (no per-lane counters).
-enum : uint32_t { kEvPrimary = 1u, kEvHit = 2u, kEvShadowed = 4u, kEvResolved = 8u, kEvFallback = 16u };
+enum : uint32_t {
+  kEvPrimary = 1u, kEvHit = 2u, kEvShadowed = 4u, kEvResolved = 8u, kEvFallback = 16u, kEvShadowRay = 32u
+};
+
+// Lane phases of k_trace.
+enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u };
+
+// A primary hit's remaining work (j3d/canvas.cpp:788-801): view-space normal
+// object_cs * (CS^-1 * n_tri), mark 0, db_id; and, for shadows, the world-space
+// hit point from the object's vertices and the direction to the light
+// (canvas.cpp:836-853).  Returns whether a shadow ray follows.
+__device__ inline bool finish_hit(const FrameParams& f, const DevObject* __restrict__ objs, const Hit& best,
+                                  bool writer, char* pp, f4& pos, f4& ld) {
+  const DevObject& ob = objs[best.obj];
+  const float* tnrm = ob.normals + 3 * (size_t)best.tri;
+  f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
+  n = mat_vec(f.cs_inv, n);
+  n = mat_vec(ob.cs, n);
+  float* pf = reinterpret_cast<float*>(pp);
+  uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
+  if (writer) {
+    *reinterpret_cast<uint8_t*>(pp) = 0;  // mark; r, g, b untouched
+    pf[1] = n.x;
+    pf[2] = n.y;
+    pf[3] = best.t;
+    pu[4] = best.tri;
+    pf[5] = best.u;
+    pf[6] = best.v;
+    pu[7] = ob.db_id;
+  }
+  if (!(f.flags & J3DG_SHADOW)) return false;
+  const uint32_t* tr = ob.idx + 3 * (size_t)best.tri;
+  f4 V[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* vv = ob.verts + 3 * (size_t)gload(tr + k);
+    V[k] = mat_vec(ob.cs, f4{gload(vv), gload(vv + 1), gload(vv + 2), 1.f});
+  }
+  const float bu = best.u, bv = best.v, s0 = 1.f - bu - bv;
+  pos = f4{V[0].x * s0 + bu * V[1].x + bv * V[2].x, V[0].y * s0 + bu * V[1].y + bv * V[2].y,
+           V[0].z * s0 + bu * V[1].z + bv * V[2].z, V[0].w * s0 + bu * V[1].w + bv * V[2].w};
+  if (f.flags & J3DG_DIRECTIONAL_LIGHT)
+    ld = f4{f.light_dir[0], f.light_dir[1], f.light_dir[2], 0.f};
+  else
+    ld = f4{f.light[0] - pos.x, f.light[1] - pos.y, f.light[2] - pos.z, f.light[3] - pos.w};
+  return true;
+}
 
 __device__ inline void add_counter(uint64_t* counters, int k, uint64_t v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -43,9 +89,11 @@ __device__ inline void add_counter(uint64_t* counters, int k, uint64_t v) {
 //   6 stack overflows  7 learned: resolved by the bounded pass  8 learned: fell back
 //   9 wave loop trips  10 lane steps
 //
-// kModeFrame: primary rays of FrameParams' rectangle.  A hit writes depth,
-//   object_id and the barycentrics and parks the object index in db_id for
-//   k_finish; a miss writes the final miss fields (j3d/canvas.cpp:859-866).
+// kModeFrame: primary rays of FrameParams' rectangle; a miss writes the miss
+//   fields (j3d/canvas.cpp:859-866).  With f.fused a hit is completed in the
+//   lane (finish_hit) and its shadow ray traced right after by the same lane,
+//   so the frame is one launch; otherwise a hit writes depth, object_id and
+//   the barycentrics and parks the object index in db_id for k_finish.
 // kModeShadow: any-hit rays from the shadow queue k_finish filled; a hit sets
 //   the pixel's mark bit 0 (j3d/canvas.cpp:855-856).
 // kModeRays / kModeAnyRays: world-space rays from a buffer (ray queries).
@@ -58,11 +106,12 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   extern __shared__ uint2 lds_stack[];
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
-  Counters c = {0, 0, 0};
-  uint32_t acc[5] = {0, 0, 0, 0, 0};  // wave-uniform event counts
+  Counters c = {0, 0, 0}, cs = {0, 0, 0};  // primary (or query) / shadow-phase work
+  uint32_t acc[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform event counts
   uint32_t trips = 0;
   uint32_t ray_steps = 0;            // counting: steps of the current ray
   uint32_t max_steps = 0, hist[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t max_steps_s = 0, hist_s[7] = {0, 0, 0, 0, 0, 0, 0};  // fused shadow phase
   bool coop = false;                 // cooperative tail mode (wave-uniform)
   // counting: per-wave timeline (shader clock at start, pool exhausted, tail mode, end; trips)
   uint64_t tl_start = COUNT ? (uint64_t)clock64() : 0, tl_exh = 0, tl_coop = 0;
@@ -74,6 +123,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   if (MODE == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
   const bool learned = MODE == kModeFrame && (f.flags & J3DG_LEARNED) && f.proposal;
   const bool any_mode = MODE == kModeAnyRays || MODE == kModeShadow;
+  const bool fused = MODE == kModeFrame && f.fused;
 
   // lane state
   uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain
@@ -134,6 +184,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
         best.found = false;
         st.sp = 0;
+        s.any = any_mode;
         if (COUNT) ray_steps = 0;
         if (n_obj) {
           begin_object(s, objs, 0, wo, wd, tn, tf, best);
@@ -175,17 +226,23 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (COUNT && coop) trips_coop++;
     if (phase != 0) {
       if (COUNT) ray_steps++;
-      bool done = n_obj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, c) : trav_step<W, D, COUNT>(s, best, st, c));
-      if (done && !(any_mode && best.found) && s.obj + 1 < n_obj) {
+      Counters& cc = (MODE == kModeFrame && phase == kShadowPhase) ? cs : c;
+      bool done = n_obj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc));
+      if (done && !(s.any && best.found) && s.obj + 1 < n_obj) {
         begin_object(s, objs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
         done = false;
       }
-      if (done && COUNT && phase != 1 && writer) {
-        // per-ray step histogram, kept in registers and reduced once at exit
-        max_steps = max(max_steps, ray_steps);
+      if (done && COUNT && phase != kBounded && writer) {
+        // per-ray step histograms (primary / shadow), kept in registers and reduced once at exit
         const uint32_t b = 31u - __clz(ray_steps | 1u);
-        hist[b > 6 ? 6 : b]++;
+        if (phase == kShadowPhase) {
+          max_steps_s = max(max_steps_s, ray_steps);
+          hist_s[b > 6 ? 6 : b]++;
+        } else {
+          max_steps = max(max_steps, ray_steps);
+          hist[b > 6 ? 6 : b]++;
+        }
       }
       if (done) {
         if (MODE == kModeAnyRays) {
@@ -200,7 +257,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             q.v[pix] = best.found ? best.v : 0.f;
           }
           phase = 0;
-        } else if (MODE == kModeShadow) {
+        } else if (MODE == kModeShadow || phase == kShadowPhase) {
           if (best.found) {
             if (writer) *(reinterpret_cast<uint8_t*>(f.gbuf) + (size_t)pix * 32) = 1;
             ev |= kEvShadowed;
@@ -222,7 +279,22 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
           float* pf = reinterpret_cast<float*>(pp);
           uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
-          if (best.found) {
+          if (best.found && fused) {
+            ev |= kEvHit;
+            f4 pos, ld;
+            if (finish_hit(f, objs, best, writer, pp, pos, ld)) {
+              // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
+              ev |= kEvShadowRay;
+              phase = kShadowPhase;
+              wo = pos;
+              wd = ld;
+              best.found = false;
+              st.sp = 0;
+              s.any = true;
+              if (COUNT) ray_steps = 0;
+              begin_object(s, objs, 0, wo, wd, 1e-3f, FLT_MAX, best);
+            }
+          } else if (best.found) {
             ev |= kEvHit;
             if (writer) {
               pf[3] = best.t;
@@ -248,14 +320,15 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       acc[2] += (uint32_t)__popcll(__ballot(ev & kEvShadowed));
       acc[3] += (uint32_t)__popcll(__ballot(ev & kEvResolved));
       acc[4] += (uint32_t)__popcll(__ballot(ev & kEvFallback));
+      acc[5] += (uint32_t)__popcll(__ballot(ev & kEvShadowRay));
     }
   }
   // node/triangle/trip/step counters: slots 4, 5, 9, 10 for primary and query
   // launches, 12..15 for the shadow launch
   const int cb = MODE == kModeShadow ? 12 : 4;
   if (lane_id() == 0) {
-    const int slot[5] = {0, 1, 3, 7, 8};
-    for (int k = 0; k < 5; ++k)
+    const int slot[6] = {0, 1, 3, 7, 8, 2};
+    for (int k = 0; k < 6; ++k)
       if (acc[k]) atomicAdd((unsigned long long*)&counters[slot[k]], (unsigned long long)acc[k]);
     if (COUNT) atomicAdd((unsigned long long*)&counters[MODE == kModeShadow ? 14 : 9], (unsigned long long)trips);
   }
@@ -269,6 +342,15 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     uint32_t mx = max_steps;
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
     if (lane_id() == 0) atomicMax((unsigned long long*)&counters[MODE == kModeShadow ? 17 : 16], (unsigned long long)mx);
+    if (fused) {  // the shadow phase's work goes to the shadow slots
+      add_counter(counters, 12, cs.nodes);
+      add_counter(counters, 13, cs.tris);
+      add_counter(counters, 15, cs.steps);
+      for (int k = 0; k < 7; ++k) add_counter(counters, 25 + k, hist_s[k]);
+      uint32_t ms = max_steps_s;
+      for (int off = 32; off >= 1; off >>= 1) ms = max(ms, (uint32_t)__shfl_xor((int)ms, off, 64));
+      if (lane_id() == 0) atomicMax((unsigned long long*)&counters[17], (unsigned long long)ms);
+    }
   }
   if (__any(st.overflow) && lane_id() == 0) atomicAdd((unsigned long long*)&counters[6], 1ull);
   if (COUNT && timeline && lane_id() == 0) {
@@ -632,20 +714,24 @@ int Ctx::ensure_shadow_queue(uint64_t rays) {
   return J3DG_OK;
 }
 
-// Primary trace, k_finish (normals, db_id, shadow queue) and the shadow trace.
-int Ctx::launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned) {
+// The trace stage.  Fused (default): one k_trace launch whose lanes complete
+// their hits and trace the shadow rays.  Split: primary trace, k_finish
+// (normals, db_id, shadow queue) and the shadow trace.
+int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, bool learned) {
   (void)learned;
   Ctx* ctx = this;
   RayQuery q;
   std::memset(&q, 0, sizeof(q));
   hipEvent_t* e = cur_ev();
+  FrameParams fp = fp_in;
+  fp.fused = (tuning.split_passes == 0 && fp.n_obj) ? 1u : 0u;
   const uint32_t rect_px = fp.n_tiles * 64u;
   HIP_TRY(hipEventRecord(e[0], stream), J3DG_ERR_LAUNCH);
   int rc = launch_trace_kernel(kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width, d_queue);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(e[1], stream), J3DG_ERR_LAUNCH);
   HIP_TRY(hipEventRecord(e[6], stream), J3DG_ERR_LAUNCH);
-  if (fp.n_obj) {
+  if (fp.n_obj && !fp.fused) {
     const bool shadow = (fp.flags & J3DG_SHADOW) != 0;
     if (shadow) {
       rc = ensure_shadow_queue(rect_px);
@@ -782,6 +868,7 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
     return ctx->fail(J3DG_ERR_INVALID_ARG, "rays_per_item must be 1, 2 or 4");
   if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256)
     return ctx->fail(J3DG_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
+  if (n.split_passes > 1) return ctx->fail(J3DG_ERR_INVALID_ARG, "split_passes must be 0 or 1");
   ctx->tuning = n;
   return J3DG_OK;
 }

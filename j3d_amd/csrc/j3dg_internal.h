@@ -42,6 +42,7 @@ struct FrameParams {
   float light_dir[3];
   uint32_t flags;
   uint32_t n_obj;
+  uint32_t fused;           // primary + hit completion + shadow ray in one launch
   j3dg_pixel* gbuf;
   const float2* proposal;   // learned: per-pixel (t bound, hit logit), indexed y * w + x
   float margin;
@@ -147,7 +148,7 @@ struct Ctx {
   hipEvent_t* cur_ev() { return ring.empty() ? ev : &ring[8 * (ring_n % kRing)]; }
   bool traced = false, shaded = false, mlp_ran = false;
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 1, 256, 8};  // stack depth, rays per item, coop tail, block, refill
+  j3dg_tuning tuning{16, 1, 1, 256, 8, 0};  // stack depth, rays per item, coop tail, block, refill, split
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   uint32_t* d_queue = nullptr;

@@ -60,14 +60,25 @@ def test_modes_icosphere(renderer, flags):
     renderer.remove_mesh(slot)
 
 
-def test_torus_shadow_directional(renderer):
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("light_kind", ["directional", "point"])
+def test_torus_shadow(renderer, split, light_kind):
+    """Shadow rays in one launch (split 0: a lane traces its pixel's shadow ray
+    after the primary hit) and in three (split 1: primary, k_finish, shadow)."""
     v, t, cam, cs, ci, pivot, diag, light = torus_scene(400, 160)
     slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
-    fr = J.make_frame(640, 360, cam, cs, ci, pivot, diag, [slot],
-                      flags=EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT, light_dir=light)
-    g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr)
-    _assert_parity(g, im, og, oim, "torus 128k tris 640x360 shadow")
+    flags = EDGES | ffi.SHADOW | (ffi.DIRECTIONAL_LIGHT if light_kind == "directional" else 0)
+    fr = J.make_frame(640, 360, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+    tun = renderer.get_tuning()
+    renderer.set_tuning(**dict(tun, split_passes=split))
+    try:
+        g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr)
+        gst = renderer.stats()
+    finally:
+        renderer.set_tuning(**tun)
+    _assert_parity(g, im, og, oim, "torus 128k tris 640x360 shadow split=%d %s" % (split, light_kind))
     assert st["shadowed"] > 0
+    assert gst["shadow_rays"] == st["shadow_rays"] and gst["shadowed"] == st["shadowed"], (gst, st)
     renderer.remove_mesh(slot)
 
 
