@@ -149,6 +149,8 @@ typedef struct j3dg_tuning {
   uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves) */
   uint32_t split_passes;   /* 0 = one launch per frame (a lane traces its pixel's shadow ray right after
                               the primary hit); 1 = primary pass, hit completion, shadow pass */
+  uint32_t cost_order;     /* 1 = hand out a target's tiles heaviest first, by the work each tile took in
+                              the target's previous frame with the same rectangle (results unchanged) */
 } j3dg_tuning;
 
 /* Statistics of the last j3dg_render / j3dg_render_batch. */
@@ -283,9 +285,9 @@ int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
    counters to HBM bytes.  *ms receives the mean duration of one copy. */
 int j3dg_calibration_copy(j3dg_ctx* ctx, uint64_t bytes, uint32_t repeats, float* ms);
 /* With counting on, the primary trace kernel of the last render records one
-   row of 8 uint64 per wave: shader clock at start, when its ray pool ran dry,
-   when it entered the cooperative tail mode (0 = never), at exit; loop trips,
-   trips in tail mode, CU id, 0. */
+   row of 8 uint64 per wave: device real-time clock (100 MHz, common to all
+   XCDs) at start, when its ray pool ran dry, when it entered the cooperative
+   tail mode (0 = never), at exit; loop trips, trips in tail mode, CU id, 0. */
 int j3dg_debug_timeline(j3dg_ctx* ctx, uint64_t* out, uint32_t max_waves, uint32_t* n_waves);
 
 #ifdef __cplusplus

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   uint32_t max_steps_s = 0, hist_s[7] = {0, 0, 0, 0, 0, 0, 0};  // fused shadow phase
   bool coop = false;                 // cooperative tail mode (wave-uniform)
   // counting: per-wave timeline (shader clock at start, pool exhausted, tail mode, end; trips)
-  uint64_t tl_start = COUNT ? (uint64_t)clock64() : 0, tl_exh = 0, tl_coop = 0;
+  uint64_t tl_start = COUNT ? (uint64_t)wall_clock64() : 0, tl_exh = 0, tl_coop = 0;
   uint32_t trips_coop = 0;
   RayPool pool;
   pool.init();
@@ -126,8 +126,9 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   const bool fused = MODE == kModeFrame && f.fused;
 
   // lane state
-  uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain
+  uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain, 3 shadow (one-launch frames)
   uint32_t pix = 0;    // pixel index (frames, shadow queue) or ray id (queries)
+  uint32_t cost_slot = 0, pix_trips = 0;  // frames: where this pixel's loop trips go, and their count
   f4 wo = {0.f, 0.f, 0.f, 1.f}, wd = {0.f, 0.f, 0.f, 0.f};
   Trav s;
   s.any = any_mode;
@@ -150,7 +151,13 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         float tn, tf;
         if (MODE == kModeFrame) {
           uint32_t px, py;
-          if (!tile_pixel(f, id >> 6, id & 63u, px, py)) continue;
+          const uint32_t tile = f.tile_order ? gload(f.tile_order + (id >> 6)) : (id >> 6);
+          cost_slot = tile * 64u + (id & 63u);
+          pix_trips = 0;
+          if (!tile_pixel(f, tile, id & 63u, px, py)) {
+            if (f.pixel_cost) f.pixel_cost[cost_slot] = 0;
+            continue;
+          }
           primary_ray(f, px, py, wo, wd);
           tn = f.diagonal / 100.f;
           tf = FLT_MAX;
@@ -197,7 +204,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     }
     if ((MODE == kModeFrame) && __ballot(ev & kEvPrimary)) acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
     ev &= ~kEvPrimary;
-    if (COUNT && pool.exhausted && !tl_exh) tl_exh = (uint64_t)clock64();
+    if (COUNT && pool.exhausted && !tl_exh) tl_exh = (uint64_t)wall_clock64();
     if (coop_tail && !coop && pool.exhausted) {
       const uint64_t act = __ballot(phase != 0);
       const uint32_t n_act = (uint32_t)__popcll(act);
@@ -211,6 +218,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         shfl_from(phase, leader);
         shfl_from(pix, leader);
         shfl_from(ray_steps, leader);
+        shfl_from(cost_slot, leader);
+        shfl_from(pix_trips, leader);
         shfl_from(wo, leader);
         shfl_from(wd, leader);
         shfl_from(s, leader);
@@ -218,7 +227,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         shfl_from(st, leader);
         if (!has) phase = 0u;
         coop = true;
-        if (COUNT) tl_coop = (uint64_t)clock64();
+        if (COUNT) tl_coop = (uint64_t)wall_clock64();
       }
     }
     const bool writer = !coop || group_lane() == 0;
@@ -226,6 +235,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (COUNT && coop) trips_coop++;
     if (phase != 0) {
       if (COUNT) ray_steps++;
+      if (MODE == kModeFrame) pix_trips++;
       Counters& cc = (MODE == kModeFrame && phase == kShadowPhase) ? cs : c;
       bool done = n_obj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc));
       if (done && !(s.any && best.found) && s.obj + 1 < n_obj) {
@@ -313,6 +323,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
     }
+    if (MODE == kModeFrame && f.pixel_cost && phase == 0 && pix_trips && writer) {
+      f.pixel_cost[cost_slot] = (uint16_t)min(pix_trips, 65535u);  // the pixel is complete
+      pix_trips = 0;
+    }
     if (!writer) ev = 0;
     if ((MODE == kModeFrame || MODE == kModeShadow) && __ballot(ev != 0)) {
       acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
@@ -356,7 +370,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   if (COUNT && timeline && lane_id() == 0) {
     const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     uint64_t* o = timeline + 8 * w;
-    o[0] = tl_start, o[1] = tl_exh, o[2] = tl_coop, o[3] = (uint64_t)clock64();
+    o[0] = tl_start, o[1] = tl_exh, o[2] = tl_coop, o[3] = (uint64_t)wall_clock64();
     o[4] = trips, o[5] = trips_coop, o[6] = (uint64_t)__smid(), o[7] = 0;
   }
 }
@@ -549,6 +563,110 @@ __global__ void k_no_scene(j3dg_pixel* g, uint64_t n) {
   g[i].object_id = kMiss;
 }
 
+// Orders a frame's tiles heaviest first for the next frame with the same
+// rectangle (longest-processing-time first: the pixels that take longest run
+// while the whole GPU is busy instead of forming the frame's tail).  A tile's
+// cost is the loop trips of its 64 pixels; tiles are binned by cost on a log2
+// scale (four bins per doubling, 64 bins) and laid out heaviest bin first, each bin in tile order so
+// that concurrently running waves stay on neighbouring tiles.  Three small
+// launches: per-block bin counts, a scan, a stable scatter.
+constexpr uint32_t kOrderBins = 64;
+constexpr uint32_t kOrderBlock = 256;  // tiles per block of k_tile_bins / k_order_scatter
+
+__device__ inline uint32_t cost_bin(uint32_t c) {
+  // 0: up to 2 trips per pixel (background); then four bins per doubling
+  if (c < 128u) return 0u;
+  const uint32_t b = 1u + (uint32_t)(4.f * __log2f((float)c * (1.f / 128.f)));
+  return b < kOrderBins ? b : kOrderBins - 1u;
+}
+
+__device__ inline void wave_bin_counts(uint32_t bin, bool valid, uint32_t* counts /* kOrderBins */) {
+  for (uint32_t b = 0; b < kOrderBins; ++b) {
+    const uint64_t m = __ballot(valid && bin == b);
+    if (lane_id() == 0) counts[b] = (uint32_t)__popcll(m);
+  }
+}
+
+__global__ void __launch_bounds__(kOrderBlock) k_tile_bins(const uint16_t* __restrict__ cost, uint32_t n_tiles,
+                                                            uint8_t* __restrict__ tile_bin,
+                                                            uint32_t* __restrict__ block_counts) {
+  __shared__ uint32_t wc[kOrderBlock / 64][kOrderBins];
+  const uint32_t i = blockIdx.x * kOrderBlock + threadIdx.x;
+  uint32_t bin = 0;
+  if (i < n_tiles) {
+    const uint4* p = reinterpret_cast<const uint4*>(cost + (size_t)i * 64);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint4 q = p[k];
+      sum += (q.x & 0xFFFFu) + (q.x >> 16) + (q.y & 0xFFFFu) + (q.y >> 16) + (q.z & 0xFFFFu) + (q.z >> 16) +
+             (q.w & 0xFFFFu) + (q.w >> 16);
+    }
+    bin = cost_bin(sum);
+    tile_bin[i] = (uint8_t)bin;
+  }
+  wave_bin_counts(bin, i < n_tiles, wc[threadIdx.x >> 6]);
+  __syncthreads();
+  if (threadIdx.x < kOrderBins) {
+    uint32_t c = 0;
+    for (uint32_t w = 0; w < kOrderBlock / 64; ++w) c += wc[w][threadIdx.x];
+    block_counts[blockIdx.x * kOrderBins + threadIdx.x] = c;
+  }
+}
+
+// In place: block_counts[block][bin] -> first output slot of (block, bin),
+// heaviest bin first: one exclusive scan over the sequence (bin 15, blocks
+// 0..n-1), (bin 14, blocks 0..n-1), ...  One block of 1024 threads, each
+// scanning a contiguous segment.
+__global__ void __launch_bounds__(1024) k_order_scan(uint32_t n_blocks, uint32_t* __restrict__ block_counts) {
+  __shared__ uint32_t part[1024];
+  const uint32_t n = n_blocks * kOrderBins, t = threadIdx.x;
+  const uint32_t seg = (n + 1023u) / 1024u, lo = min(n, t * seg), hi = min(n, lo + seg);
+  auto at = [&](uint32_t k) -> uint32_t& {  // k-th element of the heaviest-first sequence
+    const uint32_t bin = kOrderBins - 1u - k / n_blocks, blk = k % n_blocks;
+    return block_counts[blk * kOrderBins + bin];
+  };
+  uint32_t sum = 0;
+  for (uint32_t k = lo; k < hi; ++k) sum += at(k);
+  part[t] = sum;
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    __syncthreads();
+    const uint32_t o = t >= d ? part[t - d] : 0u;
+    __syncthreads();
+    part[t] += o;
+  }
+  __syncthreads();
+  uint32_t run = part[t] - sum;  // exclusive prefix of this segment
+  for (uint32_t k = lo; k < hi; ++k) {
+    uint32_t& c = at(k);
+    const uint32_t v = c;
+    c = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(kOrderBlock) k_order_scatter(const uint8_t* __restrict__ tile_bin, uint32_t n_tiles,
+                                                                const uint32_t* __restrict__ block_first,
+                                                                uint32_t* __restrict__ order) {
+  __shared__ uint32_t wc[kOrderBlock / 64][kOrderBins];
+  const uint32_t i = blockIdx.x * kOrderBlock + threadIdx.x;
+  const bool valid = i < n_tiles;
+  const uint32_t bin = valid ? tile_bin[i] : 0u;
+  const uint32_t w = threadIdx.x >> 6;
+  wave_bin_counts(bin, valid, wc[w]);
+  uint64_t same = 0;  // lanes of this wave with the same bin
+  for (uint32_t b = 0; b < kOrderBins; ++b) {
+    const uint64_t mb = __ballot(valid && bin == b);
+    if (b == bin) same = mb;
+  }
+  __syncthreads();
+  if (valid) {
+    uint32_t pos = block_first[blockIdx.x * kOrderBins + bin];
+    for (uint32_t k = 0; k < w; ++k) pos += wc[k][bin];
+    order[pos + rank_in_mask(same)] = i;
+  }
+}
+
 // A fresh proposal buffer predicts no hit anywhere (plain traversal).
 __global__ void k_clear_proposal(float2* p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -717,7 +835,7 @@ int Ctx::ensure_shadow_queue(uint64_t rays) {
 // The trace stage.  Fused (default): one k_trace launch whose lanes complete
 // their hits and trace the shadow rays.  Split: primary trace, k_finish
 // (normals, db_id, shadow queue) and the shadow trace.
-int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, bool learned) {
+int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, bool learned, Target* tg) {
   (void)learned;
   Ctx* ctx = this;
   RayQuery q;
@@ -726,9 +844,45 @@ int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, b
   FrameParams fp = fp_in;
   fp.fused = (tuning.split_passes == 0 && fp.n_obj) ? 1u : 0u;
   const uint32_t rect_px = fp.n_tiles * 64u;
+  const bool order = tg && tuning.cost_order && fp.n_obj && fp.n_tiles;
+  if (order) {
+    if (fp.n_tiles > tg->order_capacity) {
+      HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
+      if (tg->pixel_cost) hipFree(tg->pixel_cost);
+      if (tg->tile_cost) hipFree(tg->tile_cost);
+      if (tg->tile_order) hipFree(tg->tile_order);
+      tg->pixel_cost = nullptr, tg->tile_cost = nullptr, tg->tile_order = nullptr;
+      tg->order_capacity = 0;
+      tg->order_valid = false;
+      HIP_TRY(hipMalloc(&tg->pixel_cost, (size_t)fp.n_tiles * 64 * sizeof(uint16_t)), J3DG_ERR_OUT_OF_MEMORY);
+      const size_t nb = (fp.n_tiles + kOrderBlock - 1) / kOrderBlock;
+      HIP_TRY(hipMalloc(&tg->tile_cost, ((fp.n_tiles + 3) / 4 + nb * kOrderBins) * sizeof(uint32_t)),
+              J3DG_ERR_OUT_OF_MEMORY);
+      HIP_TRY(hipMalloc(&tg->tile_order, (size_t)fp.n_tiles * sizeof(uint32_t)), J3DG_ERR_OUT_OF_MEMORY);
+      tg->order_capacity = fp.n_tiles;
+    }
+    const bool same = tg->order_valid && tg->order_rect[0] == fp.x0 && tg->order_rect[1] == fp.y0 &&
+                      tg->order_rect[2] == fp.x1 && tg->order_rect[3] == fp.y1;
+    fp.tile_order = same ? tg->tile_order : nullptr;
+    fp.pixel_cost = tg->pixel_cost;
+  }
   HIP_TRY(hipEventRecord(e[0], stream), J3DG_ERR_LAUNCH);
   int rc = launch_trace_kernel(kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width, d_queue);
   if (rc) return rc;
+  if (order) {
+    // tile_cost holds the per-tile bins (bytes) followed by the per-block counts
+    const uint32_t nb = (fp.n_tiles + kOrderBlock - 1) / kOrderBlock;
+    uint8_t* bins = reinterpret_cast<uint8_t*>(tg->tile_cost);
+    uint32_t* counts = tg->tile_cost + (fp.n_tiles + 3) / 4;
+    hipLaunchKernelGGL(k_tile_bins, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint16_t*)tg->pixel_cost,
+                       fp.n_tiles, bins, counts);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, stream, nb, counts);
+    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint8_t*)bins, fp.n_tiles,
+                       (const uint32_t*)counts, tg->tile_order);
+    HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+    tg->order_valid = true;
+    tg->order_rect[0] = fp.x0, tg->order_rect[1] = fp.y0, tg->order_rect[2] = fp.x1, tg->order_rect[3] = fp.y1;
+  }
   HIP_TRY(hipEventRecord(e[1], stream), J3DG_ERR_LAUNCH);
   HIP_TRY(hipEventRecord(e[6], stream), J3DG_ERR_LAUNCH);
   if (fp.n_obj && !fp.fused) {
@@ -869,6 +1023,7 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
   if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256)
     return ctx->fail(J3DG_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
   if (n.split_passes > 1) return ctx->fail(J3DG_ERR_INVALID_ARG, "split_passes must be 0 or 1");
+  if (n.cost_order > 1) return ctx->fail(J3DG_ERR_INVALID_ARG, "cost_order must be 0 or 1");
   ctx->tuning = n;
   return J3DG_OK;
 }
@@ -1272,7 +1427,7 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
           }
         }
       }
-      rc = ctx->launch_trace(fp, std::max<uint32_t>(max_stack, 1), width, fp.proposal != nullptr);
+      rc = ctx->launch_trace(fp, std::max<uint32_t>(max_stack, 1), width, fp.proposal != nullptr, &tg);
       if (rc) return rc;
       ctx->traced = true;
     }

@@ -43,6 +43,8 @@ struct FrameParams {
   uint32_t flags;
   uint32_t n_obj;
   uint32_t fused;           // primary + hit completion + shadow ray in one launch
+  const uint32_t* tile_order;  // nullable: ray id / 64 -> tile (heaviest tiles of the last frame first)
+  uint16_t* pixel_cost;        // nullable: loop trips of each pixel, [tile * 64 + lane], for the next order
   j3dg_pixel* gbuf;
   const float2* proposal;   // learned: per-pixel (t bound, hit logit), indexed y * w + x
   float margin;
@@ -121,11 +123,24 @@ struct Target {
   j3dg_pixel* gbuf = nullptr;
   uint32_t* rgba = nullptr;
   float2* proposal = nullptr;  // learned: per-pixel (t bound, hit logit), allocated on first use
+  // cost-ordered tiles: per-pixel trips of the last frame, per-tile sums, and the
+  // order (heaviest first) valid for frames with the same rectangle
+  uint16_t* pixel_cost = nullptr;
+  uint32_t* tile_cost = nullptr;
+  uint32_t* tile_order = nullptr;
+  uint32_t order_capacity = 0;  // tiles
+  bool order_valid = false;
+  int32_t order_rect[4] = {0, 0, -1, -1};
   void release() {
     if (gbuf) hipFree(gbuf);
     if (rgba) hipFree(rgba);
     if (proposal) hipFree(proposal);
-    gbuf = nullptr, rgba = nullptr, proposal = nullptr;
+    if (pixel_cost) hipFree(pixel_cost);
+    if (tile_cost) hipFree(tile_cost);
+    if (tile_order) hipFree(tile_order);
+    gbuf = nullptr, rgba = nullptr, proposal = nullptr, pixel_cost = nullptr, tile_cost = nullptr, tile_order = nullptr;
+    order_capacity = 0;
+    order_valid = false;
     live = false;
   }
 };
@@ -148,7 +163,7 @@ struct Ctx {
   hipEvent_t* cur_ev() { return ring.empty() ? ev : &ring[8 * (ring_n % kRing)]; }
   bool traced = false, shaded = false, mlp_ran = false;
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 1, 256, 8, 0};  // stack depth, rays per item, coop tail, block, refill, split
+  j3dg_tuning tuning{16, 1, 1, 256, 8, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   uint32_t* d_queue = nullptr;
@@ -169,7 +184,7 @@ struct Ctx {
   // learned proposal for the frame's objects into fp.proposal (mlp.hip)
   int launch_mlp_frame(const FrameParams& fp, uint32_t n_obj);
   int reset_counters();  // work-queue heads and statistics counters
-  int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned);
+  int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned, Target* tg = nullptr);
   int launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
                           uint32_t max_stack, int width, uint32_t* qcounter);
   int ensure_shadow_queue(uint64_t rays);

@@ -67,7 +67,7 @@ class MeshInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("stack_depth", C.c_uint32), ("rays_per_item", C.c_uint32),
                 ("coop_tail", C.c_uint32), ("block_size", C.c_uint32), ("refill_min", C.c_uint32),
-                ("split_passes", C.c_uint32)]
+                ("split_passes", C.c_uint32), ("cost_order", C.c_uint32)]
 
 
 class Stats(C.Structure):

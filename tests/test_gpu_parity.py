@@ -96,3 +96,32 @@ def test_random_rays_closest_and_any(renderer):
     hit = renderer.any_hit([slot], rays)
     assert np.array_equal(hit, O.any_hits([m], rays))
     renderer.remove_mesh(slot)
+
+
+def test_cost_ordered_tiles_repeat_frames(renderer):
+    """Frames after the first hand out tiles heaviest first (by the previous
+    frame's per-pixel work); results must not change: same frame three times,
+    a different rectangle, then the first rectangle again."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(400, 160, seed=7)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    meshes = [O.Mesh(v, t, db_id=J.make_db_id(1, 0))]
+    mc, cav = J.make_matcap(0)
+    renderer.set_matcap(mc, cav)
+    tg = renderer.create_target(512, 288)
+    flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    try:
+        for k, rect in enumerate([None, None, None, (30, 20, 400, 250), (30, 20, 400, 250), None]):
+            fr = J.make_frame(512, 288, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light, rect=rect)
+            g, im = renderer.render(fr, tg)
+            og = np.zeros((288, 512), O.PIXEL_DTYPE)
+            og["object_id"] = 0xFFFFFFFF
+            # the oracle G-buffer carries over like the target's: render into the previous result
+            if k:
+                og = prev_og
+            og, _ = O.update_canvas(meshes, fr, gbuf=og)
+            oim = O.canvas_to_image(og, fr, mc, cav)
+            _assert_parity(g, im, og, oim, "repeat %d rect %s" % (k, rect))
+            prev_og = og
+    finally:
+        renderer.destroy_target(tg)
+        renderer.remove_mesh(slot)
