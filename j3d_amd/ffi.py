@@ -178,11 +178,13 @@ SIGNATURES = {
 _lib = None
 
 
-def load(path=LIB_PATH):
-    """Loads libj3dg.so (once) and declares every exported function."""
+def load(path=None):
+    """Loads libj3dg.so (once) and declares every exported function.  The
+    J3DG_LIBRARY environment variable names another build of it."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("J3DG_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError("libj3dg.so is not built (%s); run __graft_entry__.build()" % path)
     lib = C.CDLL(path)
