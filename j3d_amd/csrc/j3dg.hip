@@ -97,12 +97,24 @@ __device__ inline void add_counter(uint64_t* counters, int k, uint64_t v) {
 // kModeShadow: any-hit rays from the shadow queue k_finish filled; a hit sets
 //   the pixel's mark bit 0 (j3d/canvas.cpp:855-856).
 // kModeRays / kModeAnyRays: world-space rays from a buffer (ray queries).
+// kModeBatch: one-launch frames of a batch; a lane looks up its tile's frame
+//   (binary search over the frames' first tiles) and reads that frame's
+//   parameters from the device table.
+template <int MODE>
+__device__ inline const FrameParams& frame_ref(const FrameParams& f, const BatchParams& bp, uint32_t fi) {
+  if constexpr (MODE == kModeBatch)
+    return bp.frames[fi];
+  else
+    return f;
+}
+
 template <int W, int D, bool COUNT, int MODE>
 __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32_t n_rays_q,
                                                const DevObject* __restrict__ objs, uint32_t n_obj, uint2* spill,
                                                uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
                                                uint32_t batch, uint32_t refill_min, uint32_t coop_tail,
-                                               uint64_t* timeline) {
+                                               uint64_t* timeline, BatchParams bp) {
+  constexpr bool kFrameLike = MODE == kModeFrame || MODE == kModeBatch;
   extern __shared__ uint2 lds_stack[];
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
@@ -123,12 +135,15 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   if (MODE == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
   const bool learned = MODE == kModeFrame && (f.flags & J3DG_LEARNED) && f.proposal;
   const bool any_mode = MODE == kModeAnyRays || MODE == kModeShadow;
-  const bool fused = MODE == kModeFrame && f.fused;
+  const bool fused = MODE == kModeBatch || (MODE == kModeFrame && f.fused);
 
   // lane state
   uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain, 3 shadow (one-launch frames)
   uint32_t pix = 0;    // pixel index (frames, shadow queue) or ray id (queries)
   uint32_t cost_slot = 0, pix_trips = 0;  // frames: where this pixel's loop trips go, and their count
+  uint32_t fi = 0;                        // batches: the lane's frame
+  uint32_t lnobj = n_obj;                 // objects of the lane's frame
+  const DevObject* lobjs = objs;          // and their table
   f4 wo = {0.f, 0.f, 0.f, 1.f}, wd = {0.f, 0.f, 0.f, 0.f};
   Trav s;
   s.any = any_mode;
@@ -149,19 +164,34 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         uint32_t id = pool.take(phase == 0, queue, batch, n_rays);
         if (id == kNone) continue;
         float tn, tf;
-        if (MODE == kModeFrame) {
+        if (kFrameLike) {
           uint32_t px, py;
-          const uint32_t tile = f.tile_order ? gload(f.tile_order + (id >> 6)) : (id >> 6);
+          uint32_t tile = id >> 6;
+          if (MODE == kModeBatch) {
+            uint32_t lo = 0, hi = bp.n_frames;  // first_tile[lo] <= tile < first_tile[hi]
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (gload(bp.first_tile + mid) <= tile) lo = mid; else hi = mid;
+            }
+            fi = lo;
+            tile -= gload(bp.first_tile + fi);
+          }
+          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
+          if (MODE == kModeBatch) {
+            lnobj = F.n_obj;
+            lobjs = objs + F.obj_base;
+          }
+          if (MODE == kModeFrame && f.tile_order) tile = gload(f.tile_order + tile);
           cost_slot = tile * 64u + (id & 63u);
           pix_trips = 0;
-          if (!tile_pixel(f, tile, id & 63u, px, py)) {
-            if (f.pixel_cost) f.pixel_cost[cost_slot] = 0;
+          if (!tile_pixel(F, tile, id & 63u, px, py)) {
+            if (MODE == kModeFrame && f.pixel_cost) f.pixel_cost[cost_slot] = 0;
             continue;
           }
-          primary_ray(f, px, py, wo, wd);
-          tn = f.diagonal / 100.f;
+          primary_ray(F, px, py, wo, wd);
+          tn = F.diagonal / 100.f;
           tf = FLT_MAX;
-          pix = py * f.w + px;
+          pix = py * F.w + px;
           phase = 2;
           if (learned) {
             const float2 pr = f.proposal[pix];
@@ -193,8 +223,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         st.sp = 0;
         s.any = any_mode;
         if (COUNT) ray_steps = 0;
-        if (n_obj) {
-          begin_object(s, objs, 0, wo, wd, tn, tf, best);
+        if (lnobj) {
+          begin_object(s, lobjs, 0, wo, wd, tn, tf, best);
         } else {
           s.cur = kNone;
           s.r.t_near = tn;
@@ -202,7 +232,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
     }
-    if ((MODE == kModeFrame) && __ballot(ev & kEvPrimary)) acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
+    if (kFrameLike && __ballot(ev & kEvPrimary)) acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
     ev &= ~kEvPrimary;
     if (COUNT && pool.exhausted && !tl_exh) tl_exh = (uint64_t)wall_clock64();
     if (coop_tail && !coop && pool.exhausted) {
@@ -220,6 +250,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         shfl_from(ray_steps, leader);
         shfl_from(cost_slot, leader);
         shfl_from(pix_trips, leader);
+        if (MODE == kModeBatch) {  // the ray's frame travels with it
+          shfl_from(fi, leader);
+          shfl_from(lnobj, leader);
+          shfl_from(lobjs, leader);
+        }
         shfl_from(wo, leader);
         shfl_from(wd, leader);
         shfl_from(s, leader);
@@ -236,10 +271,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (phase != 0) {
       if (COUNT) ray_steps++;
       if (MODE == kModeFrame) pix_trips++;
-      Counters& cc = (MODE == kModeFrame && phase == kShadowPhase) ? cs : c;
-      bool done = n_obj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc));
-      if (done && !(s.any && best.found) && s.obj + 1 < n_obj) {
-        begin_object(s, objs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
+      Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
+      bool done = lnobj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc));
+      if (done && !(s.any && best.found) && s.obj + 1 < lnobj) {
+        begin_object(s, lobjs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
         done = false;
       }
@@ -269,7 +304,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           phase = 0;
         } else if (MODE == kModeShadow || phase == kShadowPhase) {
           if (best.found) {
-            if (writer) *(reinterpret_cast<uint8_t*>(f.gbuf) + (size_t)pix * 32) = 1;
+            if (writer) *(reinterpret_cast<uint8_t*>(frame_ref<MODE>(f, bp, fi).gbuf) + (size_t)pix * 32) = 1;
             ev |= kEvShadowed;
           }
           phase = 0;
@@ -282,17 +317,18 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           phase = 2;
           st.sp = 0;
           const float t0 = f.diagonal / 100.f, tb = s.r.t_far;
-          begin_object(s, objs, 0, wo, wd, tb >= t0 ? tb : t0, FLT_MAX, best);
+          begin_object(s, lobjs, 0, wo, wd, tb >= t0 ? tb : t0, FLT_MAX, best);
         } else {
           if (phase == 1) ev |= kEvResolved;
           phase = 0;
-          char* pp = reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32;
+          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
+          char* pp = reinterpret_cast<char*>(F.gbuf) + (size_t)pix * 32;
           float* pf = reinterpret_cast<float*>(pp);
           uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
           if (best.found && fused) {
             ev |= kEvHit;
             f4 pos, ld;
-            if (finish_hit(f, objs, best, writer, pp, pos, ld)) {
+            if (finish_hit(F, lobjs, best, writer, pp, pos, ld)) {
               // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
               ev |= kEvShadowRay;
               phase = kShadowPhase;
@@ -302,7 +338,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
               st.sp = 0;
               s.any = true;
               if (COUNT) ray_steps = 0;
-              begin_object(s, objs, 0, wo, wd, 1e-3f, FLT_MAX, best);
+              begin_object(s, lobjs, 0, wo, wd, 1e-3f, FLT_MAX, best);
             }
           } else if (best.found) {
             ev |= kEvHit;
@@ -328,7 +364,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       pix_trips = 0;
     }
     if (!writer) ev = 0;
-    if ((MODE == kModeFrame || MODE == kModeShadow) && __ballot(ev != 0)) {
+    if ((kFrameLike || MODE == kModeShadow) && __ballot(ev != 0)) {
       acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
       acc[1] += (uint32_t)__popcll(__ballot(ev & kEvHit));
       acc[2] += (uint32_t)__popcll(__ballot(ev & kEvShadowed));
@@ -463,10 +499,7 @@ __global__ void k_reset(uint32_t* queue, uint64_t* counters, uint32_t n_counters
 
 // canvas::canvas_to_image (j3d/canvas.cpp:582-670) with the background copy of
 // canvas::render_scene (canvas.cpp:895) fused in: one thread per pixel.
-__global__ void __launch_bounds__(256) k_shade(ShadeParams s) {
-  const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const uint32_t y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= s.w || y >= s.h) return;
+__device__ inline void shade_pixel(const ShadeParams& s, uint32_t x, uint32_t y) {
   const j3dg_pixel* row = s.gbuf + (size_t)y * s.w;
   const j3dg_pixel p = row[x];
   uint32_t out;
@@ -534,6 +567,22 @@ __global__ void __launch_bounds__(256) k_shade(ShadeParams s) {
     out = plain_color(p.mark, p.r, p.g, p.b, m, (flags & J3DG_SHADING) != 0, p.u, p.v);
   }
   s.rgba[(size_t)y * s.w + x] = out;
+}
+
+__global__ void __launch_bounds__(256) k_shade(ShadeParams s) {
+  const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const uint32_t y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= s.w || y >= s.h) return;
+  shade_pixel(s, x, y);
+}
+
+// The shading pass of every frame of a batch: blockIdx.z = frame.
+__global__ void __launch_bounds__(256) k_shade_batch(const ShadeParams* __restrict__ sps) {
+  const ShadeParams s = sps[blockIdx.z];
+  const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const uint32_t y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= s.w || y >= s.h) return;
+  shade_pixel(s, x, y);
 }
 
 // Plain streaming copy of known size: the reference point for scaling the
@@ -696,6 +745,7 @@ void* trace_kernel_mode(int width, int depth, bool count) {
 }
 
 void* trace_kernel(int mode, int width, int depth, bool count) {
+  if (mode == kModeBatch) return trace_kernel_mode<kModeBatch>(width, depth, count);
   if (mode == kModeRays) return trace_kernel_mode<kModeRays>(width, depth, false);
   if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, false);
   if (mode == kModeShadow) return trace_kernel_mode<kModeShadow>(width, depth, count);
@@ -781,7 +831,7 @@ uint32_t Ctx::trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const {
 }
 
 int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
-                             uint32_t max_stack, int width, uint32_t* qcounter) {
+                             uint32_t max_stack, int width, uint32_t* qcounter, const BatchParams* bpp) {
   Ctx* ctx = this;
   const uint32_t D = tuning.stack_depth;
   const uint32_t block = tuning.block_size;
@@ -801,7 +851,8 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
   uint32_t batch = 64u * tuning.rays_per_item;
   uint32_t refill = tuning.refill_min;
   uint32_t coop = tuning.coop_tail;
-  const DevObject* objs = d_objs;
+  // a batch's frames index their objects in the batch's own table
+  const DevObject* objs = mode == kModeBatch ? d_bobjs : d_objs;
   uint64_t* tl = nullptr;
   if (counting && mode == kModeFrame) {
     uint64_t waves = lanes / 64;
@@ -815,7 +866,10 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
     tl = d_timeline;
     timeline_waves = (uint32_t)waves;
   }
-  void* args[] = {&f, &rq, &nr, &objs, &no, &sp, &cap, &qu, &cnt, &batch, &refill, &coop, &tl};
+  BatchParams bp;
+  std::memset(&bp, 0, sizeof(bp));
+  if (bpp) bp = *bpp;
+  void* args[] = {&f, &rq, &nr, &objs, &no, &sp, &cap, &qu, &cnt, &batch, &refill, &coop, &tl, &bp};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;
 }
@@ -1570,13 +1624,177 @@ int j3dg_debug_timeline(j3dg_ctx* h, uint64_t* out, uint32_t max_waves, uint32_t
   return J3DG_OK;
 }
 
+// Many frames (cameras x meshes of uneven size) in one launch: per-frame
+// parameters and object tables go to device tables, one persistent k_trace
+// (kModeBatch) takes tiles of all frames from one queue, one k_shade_batch
+// shades them; reset + trace + shade are captured in a HIP graph and replayed
+// while the launch parameters are unchanged (the tables' contents may change).
+// Switched off: the batch launch faulted the GPU (illegal memory access) on
+// its first two runs.  Causes found by reading the code since: the launch
+// passed the single-frame object table instead of the batch's, and the
+// cooperative tail hand-off did not move a ray's frame (index, object table,
+// object count) to the lanes that take it over, so a writer lane could store
+// into another frame's G-buffer out of bounds.  Both are fixed below; the path
+// stays off until a GPU run has verified the fixes.
+static constexpr bool kBatchEnabled = false;
+
 int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n) {
   Ctx* ctx = reinterpret_cast<Ctx*>(h);
   CHECK_CTX();
-  (void)frames;
-  (void)targets;
-  (void)n;
-  return ctx->fail(J3DG_ERR_UNSUPPORTED, "batch rendering not built yet");
+  if (!kBatchEnabled)
+    return ctx->fail(J3DG_ERR_UNSUPPORTED, "j3dg_render_batch is disabled: its launch faulted the GPU and is under analysis");
+  if (n == 0) return J3DG_OK;
+  if (!frames || !targets) return ctx->fail(J3DG_ERR_INVALID_ARG, "null frame or target list");
+  if (n > 65536) return ctx->fail(J3DG_ERR_INVALID_ARG, "more than 65536 frames in one batch");
+  if (!ctx->d_matcap) return ctx->fail(J3DG_ERR_INVALID_ARG, "no matcap set");
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  std::vector<j3d::FrameParams> fps(n);
+  std::vector<j3d::ShadeParams> sps(n);
+  std::vector<uint32_t> first(n + 1, 0);
+  std::vector<j3d::DevObject> objs;
+  uint32_t max_stack = 1, max_w = 0, max_h = 0;
+  int width = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const j3dg_frame_desc& fd = frames[i];
+    if (targets[i] >= ctx->targets.size() || !ctx->targets[targets[i]].live)
+      return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target in batch");
+    Target& tg = ctx->targets[targets[i]];
+    if (fd.width != tg.w || fd.height != tg.h) return ctx->fail(J3DG_ERR_INVALID_ARG, "frame size differs from target size");
+    if (fd.flags & (J3DG_LEARNED | J3DG_KEEP_PROPOSAL | J3DG_NO_SCENE))
+      return ctx->fail(J3DG_ERR_UNSUPPORTED, "batches take plain frames (no learned bound, no NO_SCENE)");
+    if (fd.n_objects > J3DG_MAX_OBJECTS) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many objects");
+    int rc = j3d::make_frame_params(fd, tg, fps[i]);
+    if (rc) return ctx->fail(rc, "invalid frame in batch");
+    fps[i].fused = 1;
+    fps[i].obj_base = (uint32_t)objs.size();
+    for (uint32_t k = 0; k < fd.n_objects; ++k) {
+      const uint32_t slot = fd.objects[k];
+      if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
+      const Mesh& m = ctx->meshes[slot];
+      if (width && width != (int)m.info.width) return ctx->fail(J3DG_ERR_UNSUPPORTED, "a batch's meshes must share BVH width");
+      width = (int)m.info.width;
+      j3d::DevObject o;
+      std::memset(&o, 0, sizeof(o));
+      o.nodes = m.d_nodes, o.tris = m.d_tris, o.verts = m.d_verts, o.idx = m.d_idx, o.normals = m.d_normals;
+      std::memcpy(o.cs, m.cs, sizeof(o.cs));
+      std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
+      o.db_id = m.db_id;
+      o.width = m.info.width;
+      objs.push_back(o);
+      max_stack = std::max(max_stack, m.max_stack);
+    }
+    first[i + 1] = first[i] + fps[i].n_tiles;
+    j3d::make_shade_params(fd, tg, *ctx, sps[i]);
+    max_w = std::max(max_w, tg.w), max_h = std::max(max_h, tg.h);
+  }
+  if (!width) width = 8;
+  const uint32_t total_tiles = first[n];
+  if ((uint64_t)total_tiles * 64u >= 0xFFFFFFFFull) return ctx->fail(J3DG_ERR_INVALID_ARG, "batch has too many pixels");
+  // device tables (grown as needed; contents re-uploaded when they change)
+  if (n > ctx->bframes_cap || objs.size() > ctx->bobjs_cap) {
+    HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+    ctx->invalidate_graphs();
+    if (n > ctx->bframes_cap) {
+      if (ctx->d_bframes) hipFree(ctx->d_bframes);
+      if (ctx->d_bfirst) hipFree(ctx->d_bfirst);
+      if (ctx->d_bshade) hipFree(ctx->d_bshade);
+      ctx->d_bframes = nullptr, ctx->d_bfirst = nullptr, ctx->d_bshade = nullptr;
+      ctx->bframes_cap = 0;
+      HIP_CALL(hipMalloc(&ctx->d_bframes, n * sizeof(j3d::FrameParams)), J3DG_ERR_OUT_OF_MEMORY);
+      HIP_CALL(hipMalloc(&ctx->d_bfirst, (n + 1) * sizeof(uint32_t)), J3DG_ERR_OUT_OF_MEMORY);
+      HIP_CALL(hipMalloc(&ctx->d_bshade, n * sizeof(j3d::ShadeParams)), J3DG_ERR_OUT_OF_MEMORY);
+      ctx->bframes_cap = n;
+    }
+    if (objs.size() > ctx->bobjs_cap) {
+      if (ctx->d_bobjs) hipFree(ctx->d_bobjs);
+      ctx->d_bobjs = nullptr;
+      ctx->bobjs_cap = 0;
+      HIP_CALL(hipMalloc(&ctx->d_bobjs, std::max<size_t>(objs.size(), 1) * sizeof(j3d::DevObject)),
+               J3DG_ERR_OUT_OF_MEMORY);
+      ctx->bobjs_cap = (uint32_t)std::max<size_t>(objs.size(), 1);
+    }
+    ctx->batch_tables.clear();
+  }
+  {
+    std::vector<uint8_t> tab;
+    auto put = [&](const void* p, size_t b) {
+      const uint8_t* q = reinterpret_cast<const uint8_t*>(p);
+      tab.insert(tab.end(), q, q + b);
+    };
+    put(fps.data(), n * sizeof(j3d::FrameParams));
+    put(first.data(), (n + 1) * sizeof(uint32_t));
+    put(sps.data(), n * sizeof(j3d::ShadeParams));
+    put(objs.data(), objs.size() * sizeof(j3d::DevObject));
+    if (tab != ctx->batch_tables) {
+      HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+      HIP_CALL(hipMemcpy(ctx->d_bframes, fps.data(), n * sizeof(j3d::FrameParams), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+      HIP_CALL(hipMemcpy(ctx->d_bfirst, first.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+      HIP_CALL(hipMemcpy(ctx->d_bshade, sps.data(), n * sizeof(j3d::ShadeParams), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+      if (!objs.empty())
+        HIP_CALL(hipMemcpy(ctx->d_bobjs, objs.data(), objs.size() * sizeof(j3d::DevObject), hipMemcpyHostToDevice),
+                 J3DG_ERR_LAUNCH);
+      ctx->batch_tables.swap(tab);
+    }
+  }
+  // the trace launch's shape; the spill area must exist before capture
+  const uint32_t D = ctx->tuning.stack_depth, block = ctx->tuning.block_size;
+  const uint32_t n_rays = total_tiles * 64u;
+  const uint32_t grid = ctx->trace_grid(n_rays, D, block);
+  const uint32_t cap = max_stack > D ? max_stack - D + 1 : 1;
+  int rc = ctx->ensure_spill((uint64_t)grid * block, cap);
+  if (rc) return rc;
+  j3d::BatchParams bp;
+  bp.frames = ctx->d_bframes;
+  bp.first_tile = ctx->d_bfirst;
+  bp.n_frames = n;
+  const dim3 sgrid((max_w + 63) / 64, (max_h + 3) / 4, n);
+  struct Key {
+    void* fn;
+    uint32_t grid, block, D, n_rays, cap, n, sx, sy, rpi, refill, coop, count;
+    const void *spill, *frames, *first, *shade, *objs;
+  } key;
+  std::memset(&key, 0, sizeof(key));
+  key.fn = j3d::trace_kernel(j3d::kModeBatch, width, (int)D, ctx->counting);
+  key.grid = grid, key.block = block, key.D = D, key.n_rays = n_rays, key.cap = cap, key.n = n;
+  key.sx = sgrid.x, key.sy = sgrid.y, key.rpi = ctx->tuning.rays_per_item, key.refill = ctx->tuning.refill_min;
+  key.coop = ctx->tuning.coop_tail, key.count = ctx->counting ? 1u : 0u;
+  key.spill = ctx->spill, key.frames = ctx->d_bframes, key.first = ctx->d_bfirst, key.shade = ctx->d_bshade;
+  key.objs = ctx->d_bobjs;
+  std::vector<uint8_t> kb(reinterpret_cast<uint8_t*>(&key), reinterpret_cast<uint8_t*>(&key) + sizeof(key));
+  if (!ctx->graph_exec || kb != ctx->graph_key) {
+    ctx->invalidate_graphs();
+    HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+    HIP_CALL(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), J3DG_ERR_LAUNCH);
+    hipLaunchKernelGGL(j3d::k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->d_queue, ctx->d_counters, 32u);
+    j3d::FrameParams f0;
+    std::memset(&f0, 0, sizeof(f0));
+    j3d::RayQuery q;
+    std::memset(&q, 0, sizeof(q));
+    rc = ctx->launch_trace_kernel(j3d::kModeBatch, f0, q, n_rays, 0, max_stack, width, ctx->d_queue, &bp);
+    if (rc == J3DG_OK) hipLaunchKernelGGL(j3d::k_shade_batch, sgrid, dim3(256), 0, ctx->stream, (const j3d::ShadeParams*)ctx->d_bshade);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    HIP_CALL(ce, J3DG_ERR_LAUNCH);
+    ctx->graph = g;
+    HIP_CALL(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), J3DG_ERR_LAUNCH);
+    ctx->graph_key.swap(kb);
+  }
+  hipEvent_t* e = ctx->cur_ev();
+  HIP_CALL(hipEventRecord(e[0], ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipGraphLaunch(ctx->graph_exec, ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipEventRecord(e[1], ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipEventRecord(e[6], ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipEventRecord(e[7], ctx->stream), J3DG_ERR_LAUNCH);
+  ctx->graph_launches++;
+  ctx->traced = true, ctx->shaded = false, ctx->mlp_ran = false;
+  ctx->last_ev = e;
+  ctx->ring_flags[ctx->ring_n % Ctx::kRing] = 1u;  // trace time covers trace + shade
+  ctx->ring_n++;
+  return J3DG_OK;
 }
 
 }  // extern "C"

@@ -43,6 +43,7 @@ struct FrameParams {
   uint32_t flags;
   uint32_t n_obj;
   uint32_t fused;           // primary + hit completion + shadow ray in one launch
+  uint32_t obj_base;        // batches: index of this frame's first object in the batch's object table
   const uint32_t* tile_order;  // nullable: ray id / 64 -> tile (heaviest tiles of the last frame first)
   uint16_t* pixel_cost;        // nullable: loop trips of each pixel, [tile * 64 + lane], for the next order
   j3dg_pixel* gbuf;
@@ -50,7 +51,15 @@ struct FrameParams {
   float margin;
 };
 
-enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3 };
+enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3, kModeBatch = 4 };
+
+// Many frames in one launch (j3dg_render_batch): per-frame parameters on the
+// device and each frame's first tile in the batch's global tile numbering.
+struct BatchParams {
+  const FrameParams* frames;   // n_frames
+  const uint32_t* first_tile;  // n_frames + 1
+  uint32_t n_frames;
+};
 
 // Ray-buffer queries (j3dg_trace_rays / j3dg_any_hit_rays).
 struct RayQuery {
@@ -186,7 +195,7 @@ struct Ctx {
   int reset_counters();  // work-queue heads and statistics counters
   int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned, Target* tg = nullptr);
   int launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,
-                          uint32_t max_stack, int width, uint32_t* qcounter);
+                          uint32_t max_stack, int width, uint32_t* qcounter, const BatchParams* bp = nullptr);
   int ensure_shadow_queue(uint64_t rays);
   float4* d_shadow = nullptr;
   uint64_t shadow_capacity = 0;
@@ -195,7 +204,25 @@ struct Ctx {
   uint32_t timeline_waves = 0;
   uint32_t trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const;
   int launch_shade(const ShadeParams& sp);
-  void invalidate_graphs() {}
+  // batches (j3dg_render_batch): device tables, and the captured graph of
+  // reset + trace + shade replayed while its launch parameters are unchanged
+  FrameParams* d_bframes = nullptr;
+  uint32_t* d_bfirst = nullptr;
+  ShadeParams* d_bshade = nullptr;
+  DevObject* d_bobjs = nullptr;
+  uint32_t bframes_cap = 0, bobjs_cap = 0;
+  std::vector<uint8_t> batch_tables;  // what the device tables hold
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
+  uint64_t graph_launches = 0;
+  void invalidate_graphs() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
   void release_buffers() {
     if (d_queue) hipFree(d_queue);
     if (d_counters) hipFree(d_counters);
@@ -206,6 +233,14 @@ struct Ctx {
     d_shadow = nullptr;
     if (d_timeline) hipFree(d_timeline);
     d_timeline = nullptr;
+    invalidate_graphs();
+    if (d_bframes) hipFree(d_bframes);
+    if (d_bfirst) hipFree(d_bfirst);
+    if (d_bshade) hipFree(d_bshade);
+    if (d_bobjs) hipFree(d_bobjs);
+    d_bframes = nullptr, d_bfirst = nullptr, d_bshade = nullptr, d_bobjs = nullptr;
+    bframes_cap = bobjs_cap = 0;
+    batch_tables.clear();
     d_queue = nullptr, d_counters = nullptr, spill = nullptr, d_objs = nullptr, d_matcap = nullptr;
   }
 };
