@@ -4,10 +4,11 @@
 Workload (default): seeded displaced torus, 5000 x 2800 grid = 14,000,000
 vertices / 28,000,000 triangles; orbit camera at distance 3, 60 degree vertical
 fov, 1920 x 1080 primary rays plus one directional shadow ray per hit pixel;
-edges + matcap shading.  A step is one frame: raygen + BVH traversal + shadow rays + G-buffer write
-(k_trace, k_finish) and the shading/edge pass (k_shade), with the mesh and BVH
-already resident in HBM.  The learned-bounded frame adds the proposal network
-(k_infer, bf16 MFMA) before the primary trace, which then traverses with
+edges + matcap shading.  A step is one frame with the mesh and BVH already
+resident in HBM: one persistent k_trace launch (ray generation, primary
+traversal, hit completion into the G-buffer and the shadow ray, per lane) and
+the shading/edge pass (k_shade).  The learned-bounded frame adds the proposal
+network (k_infer, bf16 MFMA) before the trace, which then traverses with
 tmax = prediction * (1 + margin) and falls back to the unbounded traversal.
 
 value = (primary + shadow rays) / wall time per frame of the learned-bounded
@@ -16,14 +17,19 @@ the network.  Both frames' G-buffers must be identical (reported).
 
 The JSON line also carries:
   roofline      the trace kernel's algorithmic bytes per launch (node visits x
-                node bytes + triangle tests x 48 B + 32 B G-buffer write per
-                pixel, from the kernel's own per-ray counters) over its mean
+                node bytes + triangle tests x 48 B, primary and shadow, + 32 B
+                G-buffer write per pixel + 60 B of hit completion reads per
+                hit, from the kernel's own per-ray counters) over its mean
                 HIP-event duration, against 8 TB/s; `traffic` is the HBM bytes
                 per launch measured by rocprofv3 (profiles/), when present.
   cpu_baseline  the CPU oracle renderer (the project's restatement of
                 canvas::update_canvas + canvas_to_image with its own BVH and a
-                row-parallel std::thread pool) timed on this host's cores on a
-                band of rows of the same frame.
+                row-parallel std::thread pool) timed on this host's cores on
+                full frames of the same workload.
+
+--workload torus300m is the 300M-triangle case: 16 orbit cameras at
+3840 x 2160 with shadow rays, the network retrained on that mesh, oracle
+parity on a 256 x 256 crop of every camera (see multi_camera).
 """
 import argparse
 import json
@@ -46,7 +52,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k"])
+    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m"])
+    ap.add_argument("--grid", type=int, nargs=2, default=None, metavar=("NU", "NV"),
+                    help="torus300m: torus grid (default 15000 10000 = 300M triangles)")
+    ap.add_argument("--resolution", type=int, nargs=2, default=None, metavar=("W", "H"),
+                    help="torus300m: frame size (default 3840 2160)")
+    ap.add_argument("--cameras", type=int, default=16, help="torus300m: orbit cameras")
+    ap.add_argument("--crop", type=int, default=256, help="torus300m: side of the oracle parity crop")
     ap.add_argument("--width", type=int, default=8, choices=[4, 8])
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
@@ -56,7 +68,7 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="no CPU baseline, short output (for rocprofv3)")
     ap.add_argument("--rows", type=int, default=0, help="render only a centred band of this many rows")
     ap.add_argument("--no-learned", action="store_true", help="plain traversal only")
-    ap.add_argument("--train-rays", type=int, default=1 << 24)
+    ap.add_argument("--train-rays", type=int, default=None, help="default 2^24 (2^25 for torus300m)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--autotune", type=int, default=1, help="autotune stack depth / rays per item / margin")
     ap.add_argument("--calibrate", action="store_true",
@@ -149,12 +161,245 @@ def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, min_seconds):
             "ms_per_frame": dt / frames * 1e3}
 
 
+def orbit_cameras(n, seed):
+    """n orbit cameras looking at the origin: azimuth stratified over n equal
+    sectors, elevation 15..55 degrees, distance 2.6..3.6, drawn from the
+    library's seeded uniform generator."""
+    import ctypes as C
+    import j3d_amd as J
+    from j3d_amd import ffi
+    lib = ffi.load()
+    s = C.c_uint64(seed)
+    out = []
+    for i in range(n):
+        az = 2.0 * math.pi * (i + lib.j3dg_uniform01(C.byref(s))) / n
+        el = math.radians(15.0 + 40.0 * lib.j3dg_uniform01(C.byref(s)))
+        dist = 2.6 + lib.j3dg_uniform01(C.byref(s))
+        out.append(J.orbit_camera(dist, az, el))
+    return out
+
+
+def pick_crop(gbuf, side):
+    """The side x side window, on a half-overlapping grid of positions, whose
+    hit fraction is closest to 3/4: mostly surface, with some silhouette."""
+    h, w = gbuf.shape
+    side = min(side, w, h)
+    hit = (gbuf["object_id"] != 0xFFFFFFFF).astype(np.int64)
+    ii = np.zeros((h + 1, w + 1), np.int64)
+    ii[1:, 1:] = hit.cumsum(0).cumsum(1)
+    best = None
+    step = max(1, side // 2)
+    for y0 in range(0, h - side + 1, step):
+        for x0 in range(0, w - side + 1, step):
+            n = ii[y0 + side, x0 + side] - ii[y0, x0 + side] - ii[y0 + side, x0] + ii[y0, x0]
+            d = abs(n / float(side * side) - 0.75)
+            if best is None or d < best[0]:
+                best = (d, x0, y0)
+    _, x0, y0 = best
+    return (x0, y0, x0 + side - 1, y0 + side - 1)
+
+
+def fused_bytes(st, npx, node_bytes):
+    """Algorithmic bytes of one fused k_trace launch: node and triangle reads of
+    the primary and shadow rays, the 32 B G-buffer write of every pixel and 60 B
+    of hit completion reads (indices, vertices, normal) per hit."""
+    return ((st["node_visits"] + st["shadow_node_visits"]) * node_bytes +
+            (st["tri_tests"] + st["shadow_tri_tests"]) * 48 + 32 * npx + 60 * st["hits"])
+
+
+def multi_camera(a):
+    """300M-triangle workload: seeded displaced torus of 15000 x 10000 grid
+    vertices (150M vertices, 300M triangles), `cameras` orbit cameras at
+    3840 x 2160 with directional shadow rays, edges + matcap.  A step renders
+    every camera once (one persistent k_trace launch + k_shade per camera).  The
+    proposal network is retrained on this mesh.  Parity: every camera's
+    G-buffer against the oracle on a crop x crop window, and the learned
+    frames against the plain ones on every pixel."""
+    import j3d_amd as J
+    from j3d_amd import ffi
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from scenes import compare_gbuffers
+    nu, nv = a.grid or (15000, 10000)
+    w, h = a.resolution or (3840, 2160)
+    t0 = time.time()
+    v, t = J.torus(nu, nv, seed=1)
+    log("torus %dx%d: %d verts %d tris in %.1fs" % (nu, nv, len(v), len(t), time.time() - t0))
+    diag, pivot = J.scene_framing(v)
+    cam = J.camera_with_vertical_fov(60.0)
+    light = J.random_hemisphere(12)
+    flags = ffi.EDGES | ffi.SHADING | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    mc, cav = J.make_matcap(0)
+    r = J.Renderer(0)
+    t0 = time.time()
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=8)
+    info = r.mesh_info(slot)
+    log("upload: build %.2fs copy %.2fs, %d nodes, depth %d, %.2f GB on device (%.1fs)" % (
+        info["build_seconds"], info["upload_seconds"], info["n_nodes"], info["max_depth"],
+        info["device_bytes"] / 1e9, time.time() - t0))
+    r.set_matcap(mc, cav)
+    cams = orbit_cameras(a.cameras, 21)
+    frames = [J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light) for cs, ci in cams]
+    targets = [r.create_target(w, h) for _ in frames]
+    npx = w * h
+
+    def counting(frs):
+        r.set_counting(True)
+        outs, sts = [], []
+        for f, tg in zip(frs, targets):
+            g, im = r.render(f, tg)
+            sts.append(r.stats())
+            outs.append((g, im))
+        r.set_counting(False)
+        return outs, sts
+
+    def timed(frs):
+        for _ in range(a.warmup):
+            for f, tg in zip(frs, targets):
+                r.render(f, tg, readback=False)
+        r.synchronize()
+        r.reset_timings()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            for f, tg in zip(frs, targets):
+                r.render(f, tg, readback=False)
+        r.synchronize()
+        wall = time.perf_counter() - t0
+        tm = r.timings()
+        return wall, {k: float(np.mean(x)) for k, x in tm.items()}
+
+    plain, pst = counting(frames)
+    assert all(s["stack_overflow"] == 0 for s in pst)
+    rays_per_step = sum(s["primary_rays"] + s["shadow_rays"] for s in pst)
+    log("counting frames: %d rays per step, %d hit pixels" % (rays_per_step, sum(s["hits"] for s in pst)))
+    tune = None
+    if a.autotune:
+        tune = r.autotune(frames[0], targets[0], repeats=3)
+        log("autotune (plain): %s" % tune)
+    wall, tm = timed(frames)
+    plain_value = rays_per_step * a.steps / wall / 1e6
+    plain_ms = wall / a.steps * 1e3
+    log("plain: %.1f Mrays/s, %.2f ms per step" % (plain_value, plain_ms))
+    value, ms_per_step = plain_value, plain_ms
+
+    learned = None
+    if not a.no_learned:
+        n_train = a.train_rays or (1 << 25)
+        log("training the proposal network on %d rays x %d epochs ..." % (n_train, a.epochs))
+        rep_t = r.train_proposal(slot, n_rays=n_train, epochs=a.epochs, seed=7)
+        log("training: %s" % rep_t)
+        margin = 0.05
+        ltune = None
+        if a.autotune:
+            lf0 = J.make_frame(w, h, cam, cams[0][0], cams[0][1], pivot, diag, [slot], flags=flags | ffi.LEARNED,
+                               light_dir=light, margin=margin)
+            ltune = r.autotune(lf0, targets[0], repeats=3)
+            margin = ltune["margin"]
+            log("autotune (learned): %s" % ltune)
+        lframes = [J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags | ffi.LEARNED, light_dir=light,
+                                margin=margin) for cs, ci in cams]
+        louts, lsts = counting(lframes)
+        mism = 0
+        for (lg, lim), (g, im) in zip(louts, plain):
+            mism += int(sum(np.count_nonzero(lg[k].view(np.uint32) != g[k].view(np.uint32))
+                            if lg[k].dtype.itemsize == 4 else np.count_nonzero(lg[k] != g[k]) for k in lg.dtype.names))
+            mism += int(np.count_nonzero(lim != im))
+        del louts
+        lwall, ltm = timed(lframes)
+        learned = {"mrays": rays_per_step * a.steps / lwall / 1e6, "ms_per_step": lwall / a.steps * 1e3,
+                   "mlp_ms_per_frame": ltm["mlp_ms"], "trace_ms_per_frame": ltm["trace_ms"], "margin": margin,
+                   "resolved": int(sum(s["learned_resolved"] for s in lsts)),
+                   "fallback": int(sum(s["learned_fallback"] for s in lsts)),
+                   "primary_node_visits": int(sum(s["node_visits"] for s in lsts)),
+                   "plain_primary_node_visits": int(sum(s["node_visits"] for s in pst)),
+                   "mismatches_vs_plain": mism,
+                   "training": {k: rep_t[k] for k in ("steps", "loss_first", "loss_last", "train_ms", "label_ms",
+                                                        "hit_fraction", "val_hit_accuracy", "val_mean_abs_error",
+                                                        "val_resolved")},
+                   "autotune": ltune}
+        log("learned: %.1f Mrays/s, %d G-buffer/RGBA mismatches vs plain" % (learned["mrays"], mism))
+        value, ms_per_step = learned["mrays"], learned["ms_per_step"]
+
+    algo = sum(fused_bytes(s, npx, 96) for s in pst) / len(pst)
+    achieved = algo / (tm["trace_ms"] * 1e-3)
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "kernel": "k_trace (one launch: primary, hit completion, shadow)", "launch_ms": tm["trace_ms"],
+            "algorithmic_bytes_per_launch": int(algo),
+            "per_unit": "96 B per node visit, 48 B per triangle test (primary and shadow), 32 B per pixel, "
+                        "60 B per hit"}
+
+    # oracle parity on a crop of every camera, timed as the CPU baseline
+    import oracle as O
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    log("oracle BVH over %d triangles on %d threads ..." % (len(t), threads))
+    t0 = time.time()
+    m = O.Mesh(v, t, db_id=J.make_db_id(1, 0), threads=threads)
+    bvh_s = time.time() - t0
+    crops, cpu_s, cpu_rays, ties = [], 0.0, 0, 0
+    g_mism, rgba_max, rgba_px = {}, 0, 0
+    for i, (f, (g, im)) in enumerate(zip(frames, plain)):
+        x0, y0, x1, y1 = pick_crop(g, a.crop)
+        crops.append([x0, y0, x1, y1])
+        cf = J.make_frame(w, h, cam, cams[i][0], cams[i][1], pivot, diag, [slot], flags=flags, light_dir=light,
+                          rect=(x0, y0, x1, y1))
+        og = g.copy()
+        og[y0:y1 + 1, x0:x1 + 1] = np.zeros(1, O.PIXEL_DTYPE)[0]
+        og["object_id"][y0:y1 + 1, x0:x1 + 1] = 0xFFFFFFFF
+        t1 = time.time()
+        og, ost = O.update_canvas([m], cf, gbuf=og, threads=threads, count_ties=True)
+        cpu_s += time.time() - t1
+        cpu_rays += ost["primary_rays"] + ost["shadow_rays"]
+        ties += ost.get("near_ties", 0)
+        for k, n in compare_gbuffers(g[y0:y1 + 1, x0:x1 + 1], og[y0:y1 + 1, x0:x1 + 1]).items():
+            g_mism[k] = g_mism.get(k, 0) + n
+        # the shading pass over the whole frame, the crop coming from the oracle
+        oim = O.canvas_to_image(og, f, mc, cav)
+        d = np.abs(oim.view(np.uint8).astype(np.int16) - im.view(np.uint8).astype(np.int16))
+        rgba_max = max(rgba_max, int(d.max()))
+        rgba_px += int((d.reshape(im.shape + (4,)).max(axis=-1) > 0).sum())
+        log("camera %d crop %s: %s" % (i, crops[-1], {k: n for k, n in g_mism.items() if n}))
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    cpu = {"value": cpu_rays / cpu_s / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": "%d crops of %dx%d pixels (one per camera, %d primary + shadow rays), oracle update_canvas, "
+                     "%.1fs (oracle BVH build %.1fs excluded)" % (len(crops), a.crop, a.crop, cpu_rays, cpu_s, bvh_s),
+           "cpu_model": model}
+    parity = {"crops": crops, "gbuffer_mismatches": g_mism, "near_ties": ties,
+              "rgba_max_diff": rgba_max, "rgba_pixels_differing": rgba_px}
+
+    desc = {"workload": "displaced torus R=1 r=0.35, %dx%d grid, %d triangles, fBm 6 octaves; %d orbit cameras "
+                        "%dx%d, primary + directional shadow rays, edges + matcap" % (nu, nv, len(t), a.cameras, w, h),
+            "triangles": int(len(t)), "width": w, "height": h, "cameras": a.cameras, "shadow": "directional",
+            "launches": "one persistent k_trace + k_shade per camera (multi-frame batching is switched off)"}
+    out = {"metric": "Mrays/s @300M-tri %d cameras %dx%d + shadows: learned-bounded vs plain GPU traversal; "
+                     "%% HBM peak" % (a.cameras, w, h),
+           "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic", "config": desc, "roofline": roof,
+           "plain_mrays": plain_value, "plain_ms_per_step": plain_ms, "learned": learned,
+           "learned_over_plain": (learned["mrays"] / plain_value) if learned else None,
+           "autotune_plain": tune, "rays_per_step": int(rays_per_step),
+           "primary_rays": int(sum(s["primary_rays"] for s in pst)),
+           "shadow_rays": int(sum(s["shadow_rays"] for s in pst)),
+           "trace_ms_per_frame": tm["trace_ms"], "shade_ms_per_frame": tm["shade_ms"],
+           "bvh": {"width": 8, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
+                   "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
+           "tuning": r.get_tuning(), "parity": parity, "cpu_baseline": cpu}
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by image "
               "tiles or cameras with a final RGBA gather); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "torus300m":
+        return multi_camera(a)
     import j3d_amd as J
     from j3d_amd import ffi
     v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc = workload(a.workload)
@@ -216,8 +461,9 @@ def main():
 
     learned = None
     if not a.no_learned:
-        log("training the proposal network on %d rays x %d epochs ..." % (a.train_rays, a.epochs))
-        rep_t = r.train_proposal(slot, n_rays=a.train_rays, epochs=a.epochs, seed=7)
+        n_train = a.train_rays or (1 << 24)
+        log("training the proposal network on %d rays x %d epochs ..." % (n_train, a.epochs))
+        rep_t = r.train_proposal(slot, n_rays=n_train, epochs=a.epochs, seed=7)
         log("training: %s" % rep_t)
         lfr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags | ffi.LEARNED, light_dir=light,
                            rect=rect, margin=0.05)

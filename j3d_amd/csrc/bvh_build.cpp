@@ -66,8 +66,8 @@ struct Builder {
       uint32_t p = idx[i];
       box.grow(pb[p]);
       for (int k = 0; k < 3; ++k) {
-        cbox.lo[k] = std::min(cbox.lo[k], pc[3 * p + k]);
-        cbox.hi[k] = std::max(cbox.hi[k], pc[3 * p + k]);
+        cbox.lo[k] = std::min(cbox.lo[k], pc[3 * (size_t)p + k]);
+        cbox.hi[k] = std::max(cbox.hi[k], pc[3 * (size_t)p + k]);
       }
     }
   }
@@ -123,7 +123,7 @@ struct Builder {
       float scale = kBins / ext;
       float base = cbox.lo[axis];
       auto bin_of = [&](uint32_t p) {
-        int b = (int)((pc[3 * p + axis] - base) * scale);
+        int b = (int)((pc[3 * (size_t)p + axis] - base) * scale);
         return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
       };
       auto bin_range = [&](Bins& bins, uint32_t a, uint32_t b) {
@@ -177,7 +177,7 @@ struct Builder {
     if (median) {
       mid = begin + n / 2;
       std::nth_element(idx + begin, idx + mid, idx + end,
-                       [&](uint32_t x, uint32_t y) { return pc[3 * x + axis] < pc[3 * y + axis]; });
+                       [&](uint32_t x, uint32_t y) { return pc[3 * (size_t)x + axis] < pc[3 * (size_t)y + axis]; });
     }
     uint32_t l = n_nodes.fetch_add(2);
     nd.leaf = false;

@@ -132,8 +132,8 @@ void node_bounds(BuildCtx& c, uint32_t begin, uint32_t end, float* lo, float* hi
   for (uint32_t i = begin; i < end; ++i) {
     uint32_t p = c.idx[i];
     for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], c.blo[3 * p + k]);
-      hi[k] = std::max(hi[k], c.bhi[3 * p + k]);
+      lo[k] = std::min(lo[k], c.blo[3 * (size_t)p + k]);
+      hi[k] = std::max(hi[k], c.bhi[3 * (size_t)p + k]);
     }
   }
 }
@@ -151,8 +151,8 @@ void build_rec(BuildCtx& c, uint32_t node, uint32_t begin, uint32_t end, int dep
   for (uint32_t i = begin; i < end; ++i) {
     uint32_t p = c.idx[i];
     for (int k = 0; k < 3; ++k) {
-      clo[k] = std::min(clo[k], c.cen[3 * p + k]);
-      chi[k] = std::max(chi[k], c.cen[3 * p + k]);
+      clo[k] = std::min(clo[k], c.cen[3 * (size_t)p + k]);
+      chi[k] = std::max(chi[k], c.cen[3 * (size_t)p + k]);
     }
   }
   int axis = 0;
@@ -169,7 +169,7 @@ void build_rec(BuildCtx& c, uint32_t node, uint32_t begin, uint32_t end, int dep
       for (int k = 0; k < 3; ++k) bl[b][k] = FLT_MAX, bh[b][k] = -FLT_MAX;
     float scale = NB / ext;
     auto bin_of = [&](uint32_t p) {
-      int b = (int)((c.cen[3 * p + axis] - clo[axis]) * scale);
+      int b = (int)((c.cen[3 * (size_t)p + axis] - clo[axis]) * scale);
       return std::min(std::max(b, 0), NB - 1);
     };
     for (uint32_t i = begin; i < end; ++i) {
@@ -177,8 +177,8 @@ void build_rec(BuildCtx& c, uint32_t node, uint32_t begin, uint32_t end, int dep
       int b = bin_of(p);
       cnt[b]++;
       for (int k = 0; k < 3; ++k) {
-        bl[b][k] = std::min(bl[b][k], c.blo[3 * p + k]);
-        bh[b][k] = std::max(bh[b][k], c.bhi[3 * p + k]);
+        bl[b][k] = std::min(bl[b][k], c.blo[3 * (size_t)p + k]);
+        bh[b][k] = std::max(bh[b][k], c.bhi[3 * (size_t)p + k]);
       }
     }
     auto area = [](const float* l, const float* h) {
@@ -220,7 +220,7 @@ void build_rec(BuildCtx& c, uint32_t node, uint32_t begin, uint32_t end, int dep
   if (median) {
     mid = begin + n / 2;
     std::nth_element(c.idx + begin, c.idx + mid, c.idx + end,
-                     [&](uint32_t a, uint32_t b) { return c.cen[3 * a + axis] < c.cen[3 * b + axis]; });
+                     [&](uint32_t a, uint32_t b) { return c.cen[3 * (size_t)a + axis] < c.cen[3 * (size_t)b + axis]; });
   }
   uint32_t left = c.n_nodes.fetch_add(2);
   nd.a = left;
@@ -307,9 +307,9 @@ void trace_object(const orc_object& ob, uint32_t oi, const Ray& r, Best& best, b
   if (!ob.bvh) {
     for (uint64_t i = 0; i < ob.n_triangles; ++i) {
       const uint32_t* tr = ob.triangles + 3 * i;
-      const float* a = ob.vertices + 3 * tr[0];
-      const float* b = ob.vertices + 3 * tr[1];
-      const float* c = ob.vertices + 3 * tr[2];
+      const float* a = ob.vertices + 3 * (size_t)tr[0];
+      const float* b = ob.vertices + 3 * (size_t)tr[1];
+      const float* c = ob.vertices + 3 * (size_t)tr[2];
       float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
       float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
       consider((uint32_t)i, a, e1, e2);
