@@ -532,6 +532,10 @@ def main():
            "primary_tri_tests_per_ray": st["tri_tests"] / max(1, st["primary_rays"]),
            "trace_ms": trace_ms, "shadow_stage_ms": shadow_ms, "shade_ms": shade_ms,
            "shadow_node_visits_per_ray": st["shadow_node_visits"] / max(1, st["shadow_rays"]),
+           "wave_trips": int(st["wave_trips"] + st["shadow_wave_trips"]),
+           "lane_steps": int(st["lane_steps"] + st["shadow_lane_steps"]),
+           "simd_utilisation": (st["lane_steps"] + st["shadow_lane_steps"]) /
+                               max(1, 64 * (st["wave_trips"] + st["shadow_wave_trips"])),
            "bvh": {"width": a.width, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
                    "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
            "tuning": r.get_tuning()}

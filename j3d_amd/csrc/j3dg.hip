@@ -45,8 +45,10 @@ enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u };
 __device__ inline bool finish_hit(const FrameParams& f, const DevObject* __restrict__ objs, const Hit& best,
                                   bool writer, char* pp, f4& pos, f4& ld) {
   const DevObject& ob = objs[best.obj];
-  const float* tnrm = ob.normals + 3 * (size_t)best.tri;
-  f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
+  // one round trip: the triangle's three vertices and its normal in one 48-byte record
+  const float4* fr = ob.fin + 3 * (size_t)best.tri;
+  const float4 r0 = gload4(fr), r1 = gload4(fr + 1), r2 = gload4(fr + 2);
+  f4 n = {r0.w, r1.w, r2.w, 0.f};
   n = mat_vec(f.cs_inv, n);
   n = mat_vec(ob.cs, n);
   float* pf = reinterpret_cast<float*>(pp);
@@ -62,13 +64,10 @@ __device__ inline bool finish_hit(const FrameParams& f, const DevObject* __restr
     pu[7] = ob.db_id;
   }
   if (!(f.flags & J3DG_SHADOW)) return false;
-  const uint32_t* tr = ob.idx + 3 * (size_t)best.tri;
   f4 V[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float* vv = ob.verts + 3 * (size_t)gload(tr + k);
-    V[k] = mat_vec(ob.cs, f4{gload(vv), gload(vv + 1), gload(vv + 2), 1.f});
-  }
+  V[0] = mat_vec(ob.cs, f4{r0.x, r0.y, r0.z, 1.f});
+  V[1] = mat_vec(ob.cs, f4{r1.x, r1.y, r1.z, 1.f});
+  V[2] = mat_vec(ob.cs, f4{r2.x, r2.y, r2.z, 1.f});
   const float bu = best.u, bv = best.v, s0 = 1.f - bu - bv;
   pos = f4{V[0].x * s0 + bu * V[1].x + bv * V[2].x, V[0].y * s0 + bu * V[1].y + bv * V[2].y,
            V[0].z * s0 + bu * V[1].z + bv * V[2].z, V[0].w * s0 + bu * V[1].w + bv * V[2].w};
@@ -116,6 +115,19 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
                                                uint64_t* timeline, BatchParams bp) {
   constexpr bool kFrameLike = MODE == kModeFrame || MODE == kModeBatch;
   extern __shared__ uint2 lds_stack[];
+  // the object table (at most J3DG_MAX_OBJECTS entries outside batches) is read
+  // on every ray start and hit completion: keep the block's copy in LDS
+  constexpr bool kLdsObjects = MODE != kModeBatch;
+  __shared__ DevObject s_objs[kLdsObjects ? J3DG_MAX_OBJECTS : 1];
+  if (kLdsObjects) {
+    n_obj = min(n_obj, (uint32_t)J3DG_MAX_OBJECTS);
+    const uint32_t words = n_obj * (uint32_t)(sizeof(DevObject) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(objs);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s_objs);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = gload(src + i);
+    __syncthreads();
+  }
+  const DevObject* const tab = kLdsObjects ? s_objs : objs;
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
   Counters c = {0, 0, 0}, cs = {0, 0, 0};  // primary (or query) / shadow-phase work
@@ -143,7 +155,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   uint32_t cost_slot = 0, pix_trips = 0;  // frames: where this pixel's loop trips go, and their count
   uint32_t fi = 0;                        // batches: the lane's frame
   uint32_t lnobj = n_obj;                 // objects of the lane's frame
-  const DevObject* lobjs = objs;          // and their table
+  const DevObject* lobjs = tab;           // and their table
   f4 wo = {0.f, 0.f, 0.f, 1.f}, wd = {0.f, 0.f, 0.f, 0.f};
   Trav s;
   s.any = any_mode;
@@ -788,6 +800,7 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
     o.verts = m.d_verts;
     o.idx = m.d_idx;
     o.normals = m.d_normals;
+    o.fin = m.d_fin;
     std::memcpy(o.cs, m.cs, sizeof(o.cs));
     std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
     o.db_id = m.db_id;
@@ -822,7 +835,7 @@ int Ctx::upload_object_table(const DevObject* tab, uint32_t n) {
 
 uint32_t Ctx::trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const {
   const uint32_t waves_per_block = block / 64;
-  const uint32_t lds = D * block * sizeof(uint2);
+  const uint32_t lds = D * block * sizeof(uint2) + J3DG_MAX_OBJECTS * sizeof(DevObject);  // stack + object table
   const uint32_t rays_per_block = 64u * waves_per_block;
   uint32_t blocks_needed = (n_rays + rays_per_block - 1) / rays_per_block;
   // persistent: as many resident blocks as LDS and the 32 waves per CU allow
@@ -1229,10 +1242,11 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   size_t nb = bvh.nodes.size() * sizeof(uint32_t), tb = bvh.tris.size() * sizeof(float);
   size_t vb = 3 * n_vertices * sizeof(float), ib = 3 * n_triangles * sizeof(uint32_t);
   size_t mb = 3 * n_triangles * sizeof(float);
+  size_t fb = 3 * n_triangles * sizeof(float4);
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
   if (alloc((void**)&m.d_nodes, nb) != hipSuccess || alloc((void**)&m.d_tris, tb) != hipSuccess ||
       alloc((void**)&m.d_verts, vb) != hipSuccess || alloc((void**)&m.d_idx, ib) != hipSuccess ||
-      alloc((void**)&m.d_normals, mb) != hipSuccess) {
+      alloc((void**)&m.d_normals, mb) != hipSuccess || alloc((void**)&m.d_fin, fb) != hipSuccess) {
     m.release();
     return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device allocation for mesh failed");
   }
@@ -1241,13 +1255,41 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
             (!vb || hipMemcpy(m.d_verts, xyz, vb, hipMemcpyHostToDevice) == hipSuccess) &&
             (!ib || hipMemcpy(m.d_idx, tris, ib, hipMemcpyHostToDevice) == hipSuccess) &&
             (!mb || hipMemcpy(m.d_normals, m.normals_host.data(), mb, hipMemcpyHostToDevice) == hipSuccess);
+  // hit completion records, by triangle id: (v0, n.x), (v1, n.y), (v2, n.z), the
+  // same floats the G-buffer normal and the shadow origin are computed from
+  // (j3d/canvas.cpp:790-792, 844-847), packed so a hit needs one round trip
+  if (ok && n_triangles) {
+    const uint64_t chunk = std::min<uint64_t>(n_triangles, 1ull << 22);
+    std::vector<float4> buf(3 * (size_t)chunk);
+    const float* nh = m.normals_host.data();
+    const int threads = bo.threads > 0 ? bo.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    for (uint64_t c0 = 0; c0 < n_triangles && ok; c0 += chunk) {
+      const uint64_t c1 = std::min<uint64_t>(n_triangles, c0 + chunk);
+      const uint64_t per = (c1 - c0 + threads - 1) / threads;
+      std::vector<std::thread> pool;
+      for (int t = 0; t < threads; ++t) {
+        const uint64_t a = c0 + std::min<uint64_t>(c1 - c0, t * per), b = c0 + std::min<uint64_t>(c1 - c0, (t + 1) * per);
+        if (a < b)
+          pool.emplace_back([&, a, b]() {
+            for (uint64_t i = a; i < b; ++i)
+              for (int k = 0; k < 3; ++k) {
+                const float* v = xyz + 3 * (size_t)tris[3 * i + k];
+                buf[3 * (size_t)(i - c0) + k] = make_float4(v[0], v[1], v[2], nh[3 * i + k]);
+              }
+          });
+      }
+      for (auto& th : pool) th.join();
+      ok = hipMemcpy(m.d_fin + 3 * (size_t)c0, buf.data(), 3 * (c1 - c0) * sizeof(float4), hipMemcpyHostToDevice) ==
+           hipSuccess;
+    }
+  }
   if (!ok) {
     m.release();
     return ctx->fail(J3DG_ERR_LAUNCH, "mesh upload copy failed");
   }
   auto t2 = std::chrono::steady_clock::now();
   m.info.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
-  m.info.device_bytes = nb + tb + vb + ib + mb;
+  m.info.device_bytes = nb + tb + vb + ib + mb + fb;
   uint32_t s = (uint32_t)ctx->meshes.size();
   for (uint32_t i = 0; i < ctx->meshes.size(); ++i)
     if (!ctx->meshes[i].live) {
@@ -1676,6 +1718,7 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
       j3d::DevObject o;
       std::memset(&o, 0, sizeof(o));
       o.nodes = m.d_nodes, o.tris = m.d_tris, o.verts = m.d_verts, o.idx = m.d_idx, o.normals = m.d_normals;
+      o.fin = m.d_fin;
       std::memcpy(o.cs, m.cs, sizeof(o.cs));
       std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
       o.db_id = m.db_id;

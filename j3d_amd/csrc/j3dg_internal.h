@@ -20,6 +20,7 @@ struct DevObject {
   const float* verts;      // original vertices (shadow-ray origins)
   const uint32_t* idx;     // original triangles
   const float* normals;    // per-triangle normals
+  const float4* fin;       // per triangle, by id: (v0, n.x), (v1, n.y), (v2, n.z)
   float cs[16];            // object -> world
   float inv_cs[16];        // world -> object
   uint32_t db_id;
@@ -104,6 +105,7 @@ struct Mesh {
   float* d_verts = nullptr;
   uint32_t* d_idx = nullptr;
   float* d_normals = nullptr;
+  float4* d_fin = nullptr;  // hit completion records (see DevObject::fin)
   std::vector<float> normals_host;
   // learned proposal (see mlp.hip): fp32 parameters, packed bf16 fragments
   float* d_mlp = nullptr;
@@ -116,9 +118,11 @@ struct Mesh {
     if (d_verts) hipFree(d_verts);
     if (d_idx) hipFree(d_idx);
     if (d_normals) hipFree(d_normals);
+    if (d_fin) hipFree(d_fin);
     if (d_mlp) hipFree(d_mlp);
     if (d_mlp_packed) hipFree(d_mlp_packed);
     d_nodes = nullptr, d_tris = nullptr, d_verts = nullptr, d_idx = nullptr, d_normals = nullptr, d_mlp = nullptr;
+    d_fin = nullptr;
     d_mlp_packed = nullptr;
     normals_host.clear();
     has_proposal = false;

@@ -918,6 +918,7 @@ int j3dg_train_proposal(j3dg_ctx* hctx, uint32_t slot, const j3dg_train_opts* op
     ob.verts = m.d_verts;
     ob.idx = m.d_idx;
     ob.normals = m.d_normals;
+    ob.fin = m.d_fin;
     for (int i = 0; i < 16; ++i) ob.cs[i] = ob.inv_cs[i] = (i % 5 == 0) ? 1.f : 0.f;
     ob.db_id = m.db_id;
     ob.width = m.info.width;

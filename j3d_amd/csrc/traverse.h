@@ -90,43 +90,55 @@ __device__ inline Ray make_object_ray(const float* inv_cs, f4 o, f4 d, float tn,
 // Per-lane traversal stack: the top D entries live in LDS (interleaved by lane,
 // so any mix of stack pointers is bank-conflict free); older entries spill to a
 // global scratch area laid out [entry][lane] so that one wave's spills coalesce.
+// Entries are 2 x 32 bits (child reference, entry distance), accessed through
+// explicitly LDS- and global-typed pointers: a generic pointer would compile to
+// flat instructions, which wait on the vector memory counter as well and so
+// serialise every push and pop against the lane's outstanding loads and stores.
+typedef uint32_t j3d_u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) j3d_u32x2 lds_entry;
+typedef __attribute__((address_space(1))) j3d_u32x2 glb_entry;
+
 template <int D>
 struct Stack {
-  uint2* lds;        // &lds[0][tid], stride `stride`
+  lds_entry* lds;    // &lds[0][tid], stride `stride`
   uint32_t stride;   // threads per block
-  uint2* spill;      // &spill[0][global lane]
+  glb_entry* spill;  // &spill[0][global lane]
   uint64_t sstride;  // lanes of the grid
   uint32_t cap;      // spill entries per lane
   uint32_t sp;
   bool overflow;
 
   __device__ void init(uint2* lds_base, uint2* spill_base, uint32_t spill_cap) {
-    lds = lds_base + threadIdx.x;
+    lds = (lds_entry*)lds_base + threadIdx.x;
     stride = blockDim.x;
     sstride = (uint64_t)gridDim.x * blockDim.x;
-    spill = spill_base + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    spill = (glb_entry*)spill_base + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
     cap = spill_cap;
     sp = 0;
     overflow = false;
   }
-  __device__ void push(uint32_t ref, float dist) {
-    uint32_t slot = sp & (D - 1);
-    if (sp >= (uint32_t)D) {
-      uint32_t e = sp - D;
-      if (e >= cap) {  // cap comes from the builder's stack bound; never expected
+  // stores entry e (0-based from the bottom) of this lane; false on overflow
+  __device__ bool put(uint32_t e, uint32_t ref, float dist) {
+    const uint32_t slot = e & (D - 1);
+    if (e >= (uint32_t)D) {
+      if (e - D >= cap) {  // cap comes from the builder's stack bound; never expected
         overflow = true;
-        return;
+        return false;
       }
-      spill[(uint64_t)e * sstride] = lds[slot * stride];
+      spill[(uint64_t)(e - D) * sstride] = lds[slot * stride];
     }
-    lds[slot * stride] = make_uint2(ref, __float_as_uint(dist));
-    ++sp;
+    j3d_u32x2 v = {ref, __float_as_uint(dist)};
+    lds[slot * stride] = v;
+    return true;
+  }
+  __device__ void push(uint32_t ref, float dist) {
+    if (put(sp, ref, dist)) ++sp;
   }
   __device__ bool pop(uint32_t& ref, float& dist) {
     if (sp == 0) return false;
     --sp;
-    uint32_t slot = sp & (D - 1);
-    uint2 v = lds[slot * stride];
+    const uint32_t slot = sp & (D - 1);
+    const j3d_u32x2 v = lds[slot * stride];
     if (sp >= (uint32_t)D) lds[slot * stride] = spill[(uint64_t)(sp - D) * sstride];
     ref = v.x;
     dist = __uint_as_float(v.y);
@@ -440,18 +452,7 @@ __device__ inline uint32_t coop_visit(const float4* __restrict__ nodes, uint32_t
   // lanes 1..n_hit-1 push, farthest deepest: lane j goes to entry sp + n_hit - 1 - j
   if (n_hit > 1) {
     if (j >= 1 && (int)j < n_hit) {
-      const uint32_t e = st.sp + (uint32_t)(n_hit - 1) - j;
-      const uint32_t slot = e & (D - 1);
-      bool ok = true;
-      if (e >= (uint32_t)D) {
-        if (e - D >= st.cap) {
-          st.overflow = true;
-          ok = false;
-        } else {
-          st.spill[(uint64_t)(e - D) * st.sstride] = st.lds[slot * st.stride];
-        }
-      }
-      if (ok) st.lds[slot * st.stride] = make_uint2(ref, __float_as_uint(dist));
+      st.put(st.sp + (uint32_t)(n_hit - 1) - j, ref, dist);
     }
     st.sp += (uint32_t)(n_hit - 1);
   }
