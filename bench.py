@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--cameras", type=int, default=16, help="torus300m: orbit cameras")
     ap.add_argument("--crop", type=int, default=256, help="torus300m: side of the oracle parity crop")
     ap.add_argument("--width", type=int, default=8, choices=[4, 8])
+    ap.add_argument("--leaf-size", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="max triangles per BVH leaf (0: the library default, 2 for 8-wide trees)")
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--coop-tail", type=int, default=-1)
@@ -385,7 +387,7 @@ def multi_camera(a):
            "primary_rays": int(sum(s["primary_rays"] for s in pst)),
            "shadow_rays": int(sum(s["shadow_rays"] for s in pst)),
            "trace_ms_per_frame": tm["trace_ms"], "shade_ms_per_frame": tm["shade_ms"],
-           "bvh": {"width": 8, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
+           "bvh": {"width": 8, "leaf_size": 2, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
                    "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
            "tuning": r.get_tuning(), "parity": parity, "cpu_baseline": cpu}
     r.close()
@@ -414,7 +416,7 @@ def main():
         tun["coop_tail"] = a.coop_tail
     r.set_tuning(**tun)
     t0 = time.time()
-    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width)
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width, leaf_size=a.leaf_size)
     info = r.mesh_info(slot)
     log("upload: build %.2fs copy %.2fs, %d nodes, depth %d, %.2f GB on device" % (
         info["build_seconds"], info["upload_seconds"], info["n_nodes"], info["max_depth"],
@@ -536,7 +538,9 @@ def main():
            "lane_steps": int(st["lane_steps"] + st["shadow_lane_steps"]),
            "simd_utilisation": (st["lane_steps"] + st["shadow_lane_steps"]) /
                                max(1, 64 * (st["wave_trips"] + st["shadow_wave_trips"])),
-           "bvh": {"width": a.width, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
+           "bvh": {"width": a.width, "leaf_size": a.leaf_size or (2 if a.width == 8 else 4),
+                   "nodes": int(info["n_nodes"]),
+                   "depth": int(info["max_depth"]),
                    "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
            "tuning": r.get_tuning()}
 

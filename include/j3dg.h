@@ -102,7 +102,7 @@ typedef struct j3dg_frame_desc {
 /* BVH build options for j3dg_upload_mesh (0 = default). */
 typedef struct j3dg_build_opts {
   uint32_t width;     /* 4 or 8 (default) children per node */
-  uint32_t leaf_size; /* max triangles per leaf, default 4 */
+  uint32_t leaf_size; /* max triangles per leaf (1..4); 0 = 2 for width 8, 4 for width 4 */
   uint32_t threads;   /* host build threads, 0 = hardware_concurrency */
 } j3dg_build_opts;
 

@@ -266,7 +266,7 @@ class Renderer:
     def _c(self, rc, what):
         _check(self.lib, self.ctx, rc, what)
 
-    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=8, leaf_size=4, threads=0):
+    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=8, leaf_size=0, threads=0):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
         o = ffi.BuildOpts(int(width), int(leaf_size), int(threads))

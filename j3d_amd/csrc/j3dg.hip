@@ -1190,9 +1190,12 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   bo.width = 8;
   if (opts) {
     if (opts->width) bo.width = (int)opts->width;
-    if (opts->leaf_size) bo.leaf_size = (int)opts->leaf_size;
     bo.threads = (int)opts->threads;
   }
+  // default leaf size: two triangles for 8-wide trees (a leaf is then one
+  // 96-byte fetch, like a node), four for 4-wide ones
+  bo.leaf_size = bo.width == 8 ? 2 : 4;
+  if (opts && opts->leaf_size) bo.leaf_size = (int)opts->leaf_size;
   BuiltBVH bvh;
   const char* err = nullptr;
   if (!build_bvh(xyz, n_vertices, tris, n_triangles, bo, bvh, &err))
@@ -1244,7 +1247,10 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   size_t mb = 3 * n_triangles * sizeof(float);
   size_t fb = 3 * n_triangles * sizeof(float4);
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
-  if (alloc((void**)&m.d_nodes, nb) != hipSuccess || alloc((void**)&m.d_tris, tb) != hipSuccess ||
+  // one spare triangle record: the traversal fetches 96 bytes at a leaf, so a
+  // one-triangle leaf at the end of the array must still read in bounds
+  const size_t tb_alloc = tb + 3 * sizeof(float4);
+  if (alloc((void**)&m.d_nodes, nb) != hipSuccess || alloc((void**)&m.d_tris, tb_alloc) != hipSuccess ||
       alloc((void**)&m.d_verts, vb) != hipSuccess || alloc((void**)&m.d_idx, ib) != hipSuccess ||
       alloc((void**)&m.d_normals, mb) != hipSuccess || alloc((void**)&m.d_fin, fb) != hipSuccess) {
     m.release();
@@ -1252,6 +1258,7 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   }
   bool ok = hipMemcpy(m.d_nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice) == hipSuccess &&
             (!tb || hipMemcpy(m.d_tris, bvh.tris.data(), tb, hipMemcpyHostToDevice) == hipSuccess) &&
+            hipMemset(reinterpret_cast<char*>(m.d_tris) + tb, 0, tb_alloc - tb) == hipSuccess &&
             (!vb || hipMemcpy(m.d_verts, xyz, vb, hipMemcpyHostToDevice) == hipSuccess) &&
             (!ib || hipMemcpy(m.d_idx, tris, ib, hipMemcpyHostToDevice) == hipSuccess) &&
             (!mb || hipMemcpy(m.d_normals, m.normals_host.data(), mb, hipMemcpyHostToDevice) == hipSuccess);

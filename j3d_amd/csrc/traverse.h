@@ -184,6 +184,10 @@ __device__ inline void test4(uint32_t lox, uint32_t hix, uint32_t loy, uint32_t 
 // Visits one wide node: pushes the entered children far-to-near and returns the
 // nearest (kNone if no child is entered).
 template <int W, int D, bool COUNT>
+__device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const Ray& r, float thi, Stack<D>& st,
+                                       Counters& c);
+
+template <int W, int D, bool COUNT>
 __device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t node, const Ray& r, float thi,
                                       Stack<D>& st, Counters& c) {
   constexpr int kVec = W == 8 ? 6 : 4;  // 16-byte loads per node
@@ -191,6 +195,14 @@ __device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t
   float4 v[kVec];
 #pragma unroll
   for (int k = 0; k < kVec; ++k) v[k] = gload4(np + k);
+  return decode_node<W, D, COUNT>(v, r, thi, st, c);
+}
+
+// Tests the node's children whose 16-byte words are in v, pushes the entered
+// ones far-to-near and returns the nearest (kNone if none is entered).
+template <int W, int D, bool COUNT>
+__device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const Ray& r, float thi, Stack<D>& st,
+                                       Counters& c) {
   if (COUNT) c.nodes++;
   const float tlo = r.t_near * kSlackDown;
   const uint32_t ex = __float_as_uint(v[0].w);
@@ -270,47 +282,76 @@ __device__ inline void begin_object(Trav& s, const DevObject* __restrict__ objs,
 // s.cur, popping (with culling against the current bound) if needed.
 // Returns true when this object's traversal is complete (or, in any-hit mode,
 // as soon as a hit is found).
+// Tests the triangle record (a = v0 | id, e1, e2) against the lane's ray and
+// keeps the closest hit.  Returns true when an any-hit ray is done.
+template <bool COUNT>
+__device__ inline bool test_triangle(Trav& s, Hit& best, Counters& c, const float4& a, const float4& e1,
+                                     const float4& e2) {
+  if (COUNT) c.tris++;
+  float t, u, v;
+  if (intersect_triangle(s.r.ox, s.r.oy, s.r.oz, s.r.dx, s.r.dy, s.r.dz, a.x, a.y, a.z, e1.x, e1.y, e1.z, e2.x,
+                         e2.y, e2.z, s.r.t_near, s.tmax, t, u, v)) {
+    uint32_t id = __float_as_uint(a.w);
+    if (s.any) {
+      best.found = true;
+      best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+      return true;
+    }
+    bool better = !best.found || t < best.t ||
+                  (t == best.t && (s.obj < best.obj || (s.obj == best.obj && id < best.tri)));
+    if (better) {
+      best.found = true;
+      best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
+      s.tmax = t;
+      s.thi = t * kSlackUp;
+    }
+  }
+  return false;
+}
+
 template <int W, int D, bool COUNT>
 __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
   if (COUNT) c.steps++;
-  if (s.cur & kLeafBit) {
-    // all of the leaf's triangles are known up front: load them two at a time
-    const uint32_t first = s.cur & kLeafIndexMask;
-    const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
-    for (uint32_t k = 0; k < count; k += 2) {
-      const float4* tp = s.tris + (size_t)(first + k) * 3;
-      const bool two = k + 1 < count;
-      float4 a0 = gload4(tp), b0 = gload4(tp + 1), c0 = gload4(tp + 2);
-      float4 a1 = a0, b1 = b0, c1 = c0;
-      if (two) a1 = gload4(tp + 3), b1 = gload4(tp + 4), c1 = gload4(tp + 5);
+  const bool leaf = (s.cur & kLeafBit) != 0u;
+  const uint32_t first = s.cur & kLeafIndexMask;
+  const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
+  if constexpr (W == 8) {
+    // An 8-wide node and the first two triangles of a leaf are both 96 bytes:
+    // lanes at nodes and lanes at leaves share one fetch and one wait.  (The
+    // triangle array carries one spare record, so a one-triangle leaf at its
+    // end reads in bounds.)
+    const float4* p = leaf ? s.tris + (size_t)first * 3 : s.nodes + (size_t)s.cur * 6;
+    float4 v[6];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (j == 1 && !two) break;
-        const float4 a = j ? a1 : a0, e1 = j ? b1 : b0, e2 = j ? c1 : c0;
-        if (COUNT) c.tris++;
-        float t, u, v;
-        if (intersect_triangle(s.r.ox, s.r.oy, s.r.oz, s.r.dx, s.r.dy, s.r.dz, a.x, a.y, a.z, e1.x, e1.y, e1.z,
-                               e2.x, e2.y, e2.z, s.r.t_near, s.tmax, t, u, v)) {
-          uint32_t id = __float_as_uint(a.w);
-          if (s.any) {
-            best.found = true;
-            best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
-            return true;
-          }
-          bool better = !best.found || t < best.t ||
-                        (t == best.t && (s.obj < best.obj || (s.obj == best.obj && id < best.tri)));
-          if (better) {
-            best.found = true;
-            best.t = t, best.u = u, best.v = v, best.tri = id, best.obj = s.obj;
-            s.tmax = t;
-            s.thi = t * kSlackUp;
-          }
-        }
+    for (int k = 0; k < 6; ++k) v[k] = gload4(p + k);
+    if (leaf) {
+      if (test_triangle<COUNT>(s, best, c, v[0], v[1], v[2])) return true;
+      if (count > 1 && test_triangle<COUNT>(s, best, c, v[3], v[4], v[5])) return true;
+      // leaves of more than two triangles (meshes built with leaf_size > 2)
+      for (uint32_t k = 2; k < count; ++k) {
+        const float4* tp = s.tris + (size_t)(first + k) * 3;
+        if (test_triangle<COUNT>(s, best, c, gload4(tp), gload4(tp + 1), gload4(tp + 2))) return true;
       }
+      s.cur = kNone;
+    } else {
+      s.cur = decode_node<W, D, COUNT>(v, s.r, s.thi, st, c);
     }
-    s.cur = kNone;
   } else {
-    s.cur = visit_node<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
+    if (leaf) {
+      // all of the leaf's triangles are known up front: load them two at a time
+      for (uint32_t k = 0; k < count; k += 2) {
+        const float4* tp = s.tris + (size_t)(first + k) * 3;
+        const bool two = k + 1 < count;
+        float4 a0 = gload4(tp), b0 = gload4(tp + 1), c0 = gload4(tp + 2);
+        float4 a1 = a0, b1 = b0, c1 = c0;
+        if (two) a1 = gload4(tp + 3), b1 = gload4(tp + 4), c1 = gload4(tp + 5);
+        if (test_triangle<COUNT>(s, best, c, a0, b0, c0)) return true;
+        if (two && test_triangle<COUNT>(s, best, c, a1, b1, c1)) return true;
+      }
+      s.cur = kNone;
+    } else {
+      s.cur = visit_node<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
+    }
   }
   if (s.cur == kNone) {
     uint32_t ref;

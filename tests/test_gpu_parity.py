@@ -82,6 +82,32 @@ def test_torus_shadow(renderer, split, light_kind):
     renderer.remove_mesh(slot)
 
 
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("leaf_size", [1, 2, 3, 4])
+def test_leaf_sizes_and_widths(renderer, width, leaf_size):
+    """Every BVH shape the builder offers gives the oracle's frame and hits.  An
+    8-wide node and a leaf's first two triangles share one 96-byte fetch, a
+    leaf's third and fourth triangle are fetched apart, and a one-triangle
+    leaf at the end of the triangle array reads the spare record."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(300, 130, seed=5)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(2, 1), width=width, leaf_size=leaf_size)
+    m = O.Mesh(v, t, db_id=J.make_db_id(2, 1))
+    fr = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [slot], flags=EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT,
+                      light_dir=light)
+    g, im, og, oim, st = _render_both(renderer, None, [m], fr)
+    _assert_parity(g, im, og, oim, "torus 78k tris W%d leaf %d" % (width, leaf_size))
+    assert st["shadowed"] > 0
+    rays = random_rays(50000, 17, np.zeros(3), 1.4)
+    got = renderer.trace_rays([slot], rays)
+    ref = O.closest_hits([m], rays)
+    for k in ("obj", "tri"):
+        assert np.array_equal(got[k], ref[k]), k
+    for k in ("t", "u", "v"):
+        assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), k
+    assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays))
+    renderer.remove_mesh(slot)
+
+
 def test_random_rays_closest_and_any(renderer):
     v, t = J.torus(300, 120, seed=3)
     slot = renderer.upload_mesh(v, t)
