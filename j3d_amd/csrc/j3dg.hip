@@ -36,18 +36,16 @@ enum : uint32_t {
 };
 
 // Lane phases of k_trace.
-enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u };
+enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u, kFinishPhase = 4u };
 
 // A primary hit's remaining work (j3d/canvas.cpp:788-801): view-space normal
 // object_cs * (CS^-1 * n_tri), mark 0, db_id; and, for shadows, the world-space
 // hit point from the object's vertices and the direction to the light
-// (canvas.cpp:836-853).  Returns whether a shadow ray follows.
-__device__ inline bool finish_hit(const FrameParams& f, const DevObject* __restrict__ objs, const Hit& best,
-                                  bool writer, char* pp, f4& pos, f4& ld) {
-  const DevObject& ob = objs[best.obj];
-  // one round trip: the triangle's three vertices and its normal in one 48-byte record
-  const float4* fr = ob.fin + 3 * (size_t)best.tri;
-  const float4 r0 = gload4(fr), r1 = gload4(fr + 1), r2 = gload4(fr + 2);
+// (canvas.cpp:836-853).  r0..r2 is the triangle's completion record
+// (DevObject::fin): its vertices and normal.  Returns whether a shadow ray
+// follows.
+__device__ inline bool finish_hit(const FrameParams& f, const DevObject& ob, const Hit& best, bool writer, char* pp,
+                                  const float4& r0, const float4& r1, const float4& r2, f4& pos, f4& ld) {
   f4 n = {r0.w, r1.w, r2.w, 0.f};
   n = mat_vec(f.cs_inv, n);
   n = mat_vec(ob.cs, n);
@@ -284,7 +282,39 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       if (COUNT) ray_steps++;
       if (MODE == kModeFrame) pix_trips++;
       Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
-      bool done = lnobj == 0 || (coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc));
+      bool done;
+      if (lnobj == 0) {
+        done = true;
+      } else if (kFrameLike && (phase == kFinishPhase || (W == 8 && !coop))) {
+        // one fetch for every lane: a node, a leaf's triangles or a hit's completion record
+        const float4* p = phase == kFinishPhase ? lobjs[best.obj].fin + 3 * (size_t)best.tri : step_fetch(s);
+        float4 v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = gload4(p + k);
+        if (phase == kFinishPhase) {
+          done = false;
+          phase = 0;
+          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
+          char* pp = reinterpret_cast<char*>(F.gbuf) + (size_t)pix * 32;
+          f4 pos, ld;
+          if (finish_hit(F, lobjs[best.obj], best, writer, pp, v[0], v[1], v[2], pos, ld)) {
+            // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
+            ev |= kEvShadowRay;
+            phase = kShadowPhase;
+            wo = pos;
+            wd = ld;
+            best.found = false;
+            st.sp = 0;
+            s.any = true;
+            if (COUNT) ray_steps = 0;
+            begin_object(s, lobjs, 0, wo, wd, 1e-3f, FLT_MAX, best);
+          }
+        } else {
+          if constexpr (W == 8) done = trav_step_fetched<D, COUNT>(s, best, st, cc, v);
+        }
+      } else {
+        done = coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc);
+      }
       if (done && !(s.any && best.found) && s.obj + 1 < lnobj) {
         begin_object(s, lobjs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
@@ -338,20 +368,9 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           float* pf = reinterpret_cast<float*>(pp);
           uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
           if (best.found && fused) {
+            // completed on the lane's next trip, whose fetch brings the triangle's record
             ev |= kEvHit;
-            f4 pos, ld;
-            if (finish_hit(F, lobjs, best, writer, pp, pos, ld)) {
-              // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
-              ev |= kEvShadowRay;
-              phase = kShadowPhase;
-              wo = pos;
-              wd = ld;
-              best.found = false;
-              st.sp = 0;
-              s.any = true;
-              if (COUNT) ray_steps = 0;
-              begin_object(s, lobjs, 0, wo, wd, 1e-3f, FLT_MAX, best);
-            }
+            phase = kFinishPhase;
           } else if (best.found) {
             ev |= kEvHit;
             if (writer) {
@@ -1246,19 +1265,21 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   size_t vb = 3 * n_vertices * sizeof(float), ib = 3 * n_triangles * sizeof(uint32_t);
   size_t mb = 3 * n_triangles * sizeof(float);
   size_t fb = 3 * n_triangles * sizeof(float4);
+  const size_t fb_alloc = fb + 3 * sizeof(float4);  // a spare record, as for the triangles
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
   // one spare triangle record: the traversal fetches 96 bytes at a leaf, so a
   // one-triangle leaf at the end of the array must still read in bounds
   const size_t tb_alloc = tb + 3 * sizeof(float4);
   if (alloc((void**)&m.d_nodes, nb) != hipSuccess || alloc((void**)&m.d_tris, tb_alloc) != hipSuccess ||
       alloc((void**)&m.d_verts, vb) != hipSuccess || alloc((void**)&m.d_idx, ib) != hipSuccess ||
-      alloc((void**)&m.d_normals, mb) != hipSuccess || alloc((void**)&m.d_fin, fb) != hipSuccess) {
+      alloc((void**)&m.d_normals, mb) != hipSuccess || alloc((void**)&m.d_fin, fb_alloc) != hipSuccess) {
     m.release();
     return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device allocation for mesh failed");
   }
   bool ok = hipMemcpy(m.d_nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice) == hipSuccess &&
             (!tb || hipMemcpy(m.d_tris, bvh.tris.data(), tb, hipMemcpyHostToDevice) == hipSuccess) &&
             hipMemset(reinterpret_cast<char*>(m.d_tris) + tb, 0, tb_alloc - tb) == hipSuccess &&
+            hipMemset(reinterpret_cast<char*>(m.d_fin) + fb, 0, fb_alloc - fb) == hipSuccess &&
             (!vb || hipMemcpy(m.d_verts, xyz, vb, hipMemcpyHostToDevice) == hipSuccess) &&
             (!ib || hipMemcpy(m.d_idx, tris, ib, hipMemcpyHostToDevice) == hipSuccess) &&
             (!mb || hipMemcpy(m.d_normals, m.normals_host.data(), mb, hipMemcpyHostToDevice) == hipSuccess);

@@ -309,34 +309,64 @@ __device__ inline bool test_triangle(Trav& s, Hit& best, Counters& c, const floa
   return false;
 }
 
+// Pops entries (culled against the current bound) until one is entered.
+// Returns true when the stack runs dry: this object's traversal is complete.
+template <int D>
+__device__ inline bool pop_next(Trav& s, Stack<D>& st) {
+  uint32_t ref;
+  float dist;
+  while (st.pop(ref, dist)) {
+    if (dist <= s.thi) {
+      s.cur = ref;
+      return false;
+    }
+  }
+  return true;
+}
+
+// 8-wide trees: an 8-wide node and the first two triangles of a leaf are both
+// 96 bytes, so lanes at nodes and lanes at leaves (and, in the frame kernel,
+// lanes completing a hit: a 48-byte record) share one fetch of six 16-byte
+// words and one wait.  The triangle array carries one spare record, so a
+// one-triangle leaf at its end reads in bounds.
+__device__ inline const float4* step_fetch(const Trav& s) {
+  return (s.cur & kLeafBit) ? s.tris + (size_t)(s.cur & kLeafIndexMask) * 3 : s.nodes + (size_t)s.cur * 6;
+}
+
+// One traversal step of an 8-wide tree on the six words step_fetch pointed to.
+template <int D, bool COUNT>
+__device__ inline bool trav_step_fetched(Trav& s, Hit& best, Stack<D>& st, Counters& c, const float4 (&v)[6]) {
+  if (COUNT) c.steps++;
+  if (s.cur & kLeafBit) {
+    const uint32_t first = s.cur & kLeafIndexMask;
+    const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
+    if (test_triangle<COUNT>(s, best, c, v[0], v[1], v[2])) return true;
+    if (count > 1 && test_triangle<COUNT>(s, best, c, v[3], v[4], v[5])) return true;
+    // leaves of more than two triangles (meshes built with leaf_size > 2)
+    for (uint32_t k = 2; k < count; ++k) {
+      const float4* tp = s.tris + (size_t)(first + k) * 3;
+      if (test_triangle<COUNT>(s, best, c, gload4(tp), gload4(tp + 1), gload4(tp + 2))) return true;
+    }
+    s.cur = kNone;
+  } else {
+    s.cur = decode_node<8, D, COUNT>(v, s.r, s.thi, st, c);
+  }
+  return s.cur == kNone ? pop_next<D>(s, st) : false;
+}
+
 template <int W, int D, bool COUNT>
 __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
-  if (COUNT) c.steps++;
   const bool leaf = (s.cur & kLeafBit) != 0u;
   const uint32_t first = s.cur & kLeafIndexMask;
   const uint32_t count = ((s.cur >> kLeafCountShift) & 3u) + 1u;
   if constexpr (W == 8) {
-    // An 8-wide node and the first two triangles of a leaf are both 96 bytes:
-    // lanes at nodes and lanes at leaves share one fetch and one wait.  (The
-    // triangle array carries one spare record, so a one-triangle leaf at its
-    // end reads in bounds.)
-    const float4* p = leaf ? s.tris + (size_t)first * 3 : s.nodes + (size_t)s.cur * 6;
+    const float4* p = step_fetch(s);
     float4 v[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = gload4(p + k);
-    if (leaf) {
-      if (test_triangle<COUNT>(s, best, c, v[0], v[1], v[2])) return true;
-      if (count > 1 && test_triangle<COUNT>(s, best, c, v[3], v[4], v[5])) return true;
-      // leaves of more than two triangles (meshes built with leaf_size > 2)
-      for (uint32_t k = 2; k < count; ++k) {
-        const float4* tp = s.tris + (size_t)(first + k) * 3;
-        if (test_triangle<COUNT>(s, best, c, gload4(tp), gload4(tp + 1), gload4(tp + 2))) return true;
-      }
-      s.cur = kNone;
-    } else {
-      s.cur = decode_node<W, D, COUNT>(v, s.r, s.thi, st, c);
-    }
+    return trav_step_fetched<D, COUNT>(s, best, st, c, v);
   } else {
+    if (COUNT) c.steps++;
     if (leaf) {
       // all of the leaf's triangles are known up front: load them two at a time
       for (uint32_t k = 0; k < count; k += 2) {
@@ -352,19 +382,8 @@ __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
     } else {
       s.cur = visit_node<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
     }
+    return s.cur == kNone ? pop_next<D>(s, st) : false;
   }
-  if (s.cur == kNone) {
-    uint32_t ref;
-    float dist;
-    while (st.pop(ref, dist)) {
-      if (dist <= s.thi) {
-        s.cur = ref;
-        return false;
-      }
-    }
-    return true;
-  }
-  return false;
 }
 
 // Wave helpers
