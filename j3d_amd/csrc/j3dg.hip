@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (!pool.exhausted && (n_idle >= refill_min || n_idle == 64)) {
       // hand new rays to idle lanes; ids outside the rectangle leave a lane idle, so repeat
       while (__ballot(phase == 0) && !pool.exhausted) {
-        uint32_t id = pool.take(phase == 0, queue, batch, n_rays);
+        uint32_t id = MODE == kModeFrame ? pool.take_banded(phase == 0, queue, batch, f.n_tiles)
+                                         : pool.take(phase == 0, queue, batch, n_rays);
         if (id == kNone) continue;
         float tn, tf;
         if (kFrameLike) {
@@ -524,7 +525,7 @@ __global__ void __launch_bounds__(256) k_finish(FrameParams f, const DevObject* 
 
 __global__ void k_reset(uint32_t* queue, uint64_t* counters, uint32_t n_counters) {
   uint32_t i = threadIdx.x;
-  if (i < 16) queue[i] = 0u;
+  for (uint32_t k = i; k < kQueueWords; k += blockDim.x) queue[k] = 0u;
   if (i < n_counters) counters[i] = 0ull;
 }
 
@@ -694,16 +695,22 @@ __global__ void __launch_bounds__(kOrderBlock) k_tile_bins(const uint16_t* __res
   }
 }
 
-// In place: block_counts[block][bin] -> first output slot of (block, bin),
-// heaviest bin first: one exclusive scan over the sequence (bin 15, blocks
-// 0..n-1), (bin 14, blocks 0..n-1), ...  One block of 1024 threads, each
-// scanning a contiguous segment.
+// In place: block_counts[block][bin] -> first output slot of (block, bin):
+// one exclusive scan over the sequence band 0 (heaviest bin: its blocks in
+// order, next bin: ...), band 1, ...  The bands are the k_trace work bands
+// (band_first_ray), so every band keeps its own tiles, heaviest first.  One
+// block of 1024 threads, each scanning a contiguous segment.
+static_assert(kOrderBlock == kBandBlock, "the sort's blocks are the bands' units");
 __global__ void __launch_bounds__(1024) k_order_scan(uint32_t n_blocks, uint32_t* __restrict__ block_counts) {
   __shared__ uint32_t part[1024];
   const uint32_t n = n_blocks * kOrderBins, t = threadIdx.x;
   const uint32_t seg = (n + 1023u) / 1024u, lo = min(n, t * seg), hi = min(n, lo + seg);
-  auto at = [&](uint32_t k) -> uint32_t& {  // k-th element of the heaviest-first sequence
-    const uint32_t bin = kOrderBins - 1u - k / n_blocks, blk = k % n_blocks;
+  auto at = [&](uint32_t k) -> uint32_t& {  // k-th element of the banded heaviest-first sequence
+    uint32_t b = 0;
+    while (b + 1 < kBands && k >= ((b + 1) * n_blocks / kBands) * kOrderBins) ++b;
+    const uint32_t b0 = b * n_blocks / kBands, nbb = (b + 1) * n_blocks / kBands - b0;
+    const uint32_t kk = k - b0 * kOrderBins;
+    const uint32_t bin = kOrderBins - 1u - kk / nbb, blk = b0 + kk % nbb;
     return block_counts[blk * kOrderBins + bin];
   };
   uint32_t sum = 0;
@@ -1058,7 +1065,7 @@ int j3dg_create(int device, j3dg_ctx** out) {
   ctx->ring.assign(8 * Ctx::kRing, nullptr);
   for (auto& e : ctx->ring)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
-  if (hipMalloc(&ctx->d_queue, 64) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->d_queue, kQueueWords * sizeof(uint32_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
   if (hipMalloc(&ctx->d_counters, 32 * sizeof(uint64_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
   *out = reinterpret_cast<j3dg_ctx*>(ctx);
   return J3DG_OK;

@@ -392,13 +392,71 @@ __device__ inline uint32_t rank_in_mask(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Frames split their tile sequence into kBands bands of consecutive
+// kBandBlock-tile blocks, one per XCD: workgroups are dispatched to the eight
+// XCDs round robin, so block b starts on band b % kBands, and each XCD's L2
+// serves the rays of one compact image band.  A wave whose band is drained
+// moves on to the next band.  Each band has its own queue counter, 64 bytes
+// apart, from word kBandQueue of the queue array.
+constexpr uint32_t kBands = 8;
+constexpr uint32_t kBandBlock = 256;
+constexpr uint32_t kBandQueue = 32;
+constexpr uint32_t kQueueWords = kBandQueue + 16 * kBands;
+
+// First ray id (tile position * 64) of band b of a frame of n_tiles tiles.
+__host__ __device__ inline uint32_t band_first_ray(uint32_t b, uint32_t n_tiles) {
+  const uint32_t nb = (n_tiles + kBandBlock - 1) / kBandBlock;
+  const uint32_t t = (b * nb / kBands) * kBandBlock;
+  return (t < n_tiles ? t : n_tiles) * 64u;
+}
+
 // Wave-local pool of ray ids taken from a global atomic queue in batches.
 struct RayPool {
   uint32_t next, end;
   bool exhausted;
+  uint32_t band, tried;  // banded frames: current band, bands drained so far
   __device__ void init() {
     next = end = 0;
     exhausted = false;
+    band = blockIdx.x % kBands;
+    tried = 0;
+  }
+  // Refills [next, end) from the frame's bands (all lanes must call).
+  __device__ void refill_banded(uint32_t* queue, uint32_t batch, uint32_t n_tiles) {
+    while (tried < kBands) {
+      const uint32_t lo = band_first_ray(band, n_tiles), hi = band_first_ray(band + 1, n_tiles);
+      uint32_t base = 0;
+      if (lane_id() == 0) base = atomicAdd(queue + kBandQueue + 16 * band, batch);
+      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+      if (base < hi - lo) {
+        next = lo + base;
+        end = min(next + batch, hi);
+        return;
+      }
+      band = band + 1 == kBands ? 0 : band + 1;
+      ++tried;
+    }
+    exhausted = true;
+  }
+  // As take, for a frame's banded tile sequence.
+  __device__ uint32_t take_banded(bool want, uint32_t* queue, uint32_t batch, uint32_t n_tiles) {
+    uint32_t id = kNone;
+    uint64_t m = __ballot(want);
+    while (m && !exhausted) {
+      if (next >= end) {
+        refill_banded(queue, batch, n_tiles);
+        if (exhausted) break;
+      }
+      uint32_t avail = end - next;
+      uint32_t r = rank_in_mask(m);
+      bool mine = ((m >> lane_id()) & 1ull) && r < avail;
+      if (mine) id = next + r;
+      uint32_t n = (uint32_t)__popcll(m);
+      uint32_t used = n < avail ? n : avail;
+      next += used;
+      m &= ~__ballot(mine);
+    }
+    return id;
   }
   // Hands ray ids to the lanes with want == true (all lanes must call).
   // Returns this lane's id, or kNone.

@@ -124,6 +124,38 @@ def test_random_rays_closest_and_any(renderer):
     renderer.remove_mesh(slot)
 
 
+def test_banded_tiles_large_frame(renderer):
+    """A frame's tiles are split into eight bands with a queue each (one per
+    XCD), each band sorted heaviest first on repeat frames: a 1280x720 frame
+    (14,400 tiles, 57 sort blocks, bands of 7-8 blocks) rendered twice, the
+    second time in the banded cost order, and a 1000x600 frame whose last
+    block is partial; every pixel must come out once and right."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(500, 200, seed=9)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    meshes = [O.Mesh(v, t, db_id=J.make_db_id(1, 0))]
+    mc, cav = J.make_matcap(0)
+    renderer.set_matcap(mc, cav)
+    flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    try:
+        for w, h in ((1280, 720), (1000, 600)):
+            fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+            tg = renderer.create_target(w, h)
+            try:
+                renderer.render(fr, tg, readback=False)
+                # stale contents a skipped tile would leave behind
+                junk = np.zeros((h, w), ffi.PIXEL_DTYPE)
+                junk["object_id"] = 7
+                junk["depth"] = 1.0
+                renderer.write_gbuffer(tg, junk)
+                g, im = renderer.render(fr, tg)
+            finally:
+                renderer.destroy_target(tg)
+            og, oim, _ = O.render(meshes, fr, mc, cav)
+            _assert_parity(g, im, og, oim, "banded %dx%d" % (w, h))
+    finally:
+        renderer.remove_mesh(slot)
+
+
 def test_cost_ordered_tiles_repeat_frames(renderer):
     """Frames after the first hand out tiles heaviest first (by the previous
     frame's per-pixel work); results must not change: same frame three times,
