@@ -394,11 +394,13 @@ __device__ inline uint32_t rank_in_mask(uint64_t m) {
 
 // Frames split their tile sequence into kBands bands of consecutive
 // kBandBlock-tile blocks, one per XCD: workgroups are dispatched to the eight
-// XCDs round robin, so block b starts on band b % kBands, and each XCD's L2
+// XCDs round robin, so block b starts on band b % 8, and each XCD's L2
 // serves the rays of one compact image band.  A wave whose band is drained
 // moves on to the next band.  Each band has its own queue counter, 64 bytes
 // apart, from word kBandQueue of the queue array.
-constexpr uint32_t kBands = 8;
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kBandsPerXcd = 1;  // 2 and 4 (an XCD draining its bands in turn) measured slower
+constexpr uint32_t kBands = kXcds * kBandsPerXcd;
 constexpr uint32_t kBandBlock = 256;
 constexpr uint32_t kBandQueue = 32;
 constexpr uint32_t kQueueWords = kBandQueue + 16 * kBands;
@@ -418,7 +420,7 @@ struct RayPool {
   __device__ void init() {
     next = end = 0;
     exhausted = false;
-    band = blockIdx.x % kBands;
+    band = (blockIdx.x % kXcds) * kBandsPerXcd;
     tried = 0;
   }
   // Refills [next, end) from the frame's bands (all lanes must call).
