@@ -18,8 +18,8 @@ the network.  Both frames' G-buffers must be identical (reported).
 The JSON line also carries:
   roofline      the trace kernel's algorithmic bytes per launch (node visits x
                 node bytes + triangle tests x 48 B, primary and shadow, + 32 B
-                G-buffer write per pixel + 60 B of hit completion reads per
-                hit, from the kernel's own per-ray counters) over its mean
+                G-buffer write per pixel + the 48 B completion record per hit,
+                from the kernel's own per-ray counters) over its mean
                 HIP-event duration, against 8 TB/s; `traffic` is the HBM bytes
                 per launch measured by rocprofv3 (profiles/), when present.
   cpu_baseline  the CPU oracle renderer (the project's restatement of
@@ -201,12 +201,16 @@ def pick_crop(gbuf, side):
     return (x0, y0, x0 + side - 1, y0 + side - 1)
 
 
+FUSED_PER_UNIT = ("%d B per node visit, 48 B per triangle test (primary and shadow), 32 B G-buffer write per pixel, "
+                  "48 B hit completion record per hit")
+
+
 def fused_bytes(st, npx, node_bytes):
     """Algorithmic bytes of one fused k_trace launch: node and triangle reads of
-    the primary and shadow rays, the 32 B G-buffer write of every pixel and 60 B
-    of hit completion reads (indices, vertices, normal) per hit."""
+    the primary and shadow rays, the 32 B G-buffer write of every pixel and the
+    48 B completion record (vertices, normal) of every hit."""
     return ((st["node_visits"] + st["shadow_node_visits"]) * node_bytes +
-            (st["tri_tests"] + st["shadow_tri_tests"]) * 48 + 32 * npx + 60 * st["hits"])
+            (st["tri_tests"] + st["shadow_tri_tests"]) * 48 + 32 * npx + 48 * st["hits"])
 
 
 def multi_camera(a):
@@ -328,8 +332,7 @@ def multi_camera(a):
             "frac": achieved / HBM_PEAK, "traffic": None,
             "kernel": "k_trace (one launch: primary, hit completion, shadow)", "launch_ms": tm["trace_ms"],
             "algorithmic_bytes_per_launch": int(algo),
-            "per_unit": "96 B per node visit, 48 B per triangle test (primary and shadow), 32 B per pixel, "
-                        "60 B per hit"}
+            "per_unit": FUSED_PER_UNIT % 96}
 
     # oracle parity on a crop of every camera, timed as the CPU baseline
     import oracle as O
@@ -498,13 +501,8 @@ def main():
     npx = w * h
     fused = r.get_tuning()["split_passes"] == 0
     if fused:
-        # one k_trace launch: primary and shadow node / triangle reads, 32 B G-buffer
-        # write per pixel, and per hit 60 B of hit completion reads (indices,
-        # vertices, normal)
-        algo_bytes = ((st["node_visits"] + st["shadow_node_visits"]) * node_bytes +
-                      (st["tri_tests"] + st["shadow_tri_tests"]) * 48 + 32 * npx + 60 * st["hits"])
-        per_unit = "%d B per node visit, 48 B per triangle test (primary and shadow), 32 B per pixel, " \
-                   "60 B per hit" % node_bytes
+        algo_bytes = fused_bytes(st, npx, node_bytes)
+        per_unit = FUSED_PER_UNIT % node_bytes
         kname = "k_trace (one launch: primary, hit completion, shadow)"
     else:
         # primary k_trace launch: node and triangle reads + 20 B of G-buffer writes per pixel
