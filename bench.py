@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m"])
+    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m", "set16x64"])
     ap.add_argument("--grid", type=int, nargs=2, default=None, metavar=("NU", "NV"),
                     help="torus300m: torus grid (default 15000 10000 = 300M triangles)")
     ap.add_argument("--resolution", type=int, nargs=2, default=None, metavar=("W", "H"),
@@ -397,6 +397,182 @@ def multi_camera(a):
     print(json.dumps(out), flush=True)
 
 
+def mesh_set(n_meshes, seed):
+    """n displaced tori (R=1, r=0.35, fBm) whose triangle counts are drawn
+    log-uniform in [1e4, 1e7]: grid nu x nv with nu/nv ~ R/r, 2 nu nv triangles."""
+    import ctypes as C
+    import j3d_amd as J
+    from j3d_amd import ffi
+    lib = ffi.load()
+    s = C.c_uint64(seed)
+    out = []
+    for i in range(n_meshes):
+        target = 10.0 ** (4.0 + 3.0 * lib.j3dg_uniform01(C.byref(s)))
+        nv = max(3, int(round(math.sqrt(target / 2.0 / 2.857))))
+        nu = max(3, int(round(target / 2.0 / nv)))
+        v, t = J.torus(nu, nv, seed=100 + i)
+        out.append((nu, nv, v, t))
+    return out
+
+
+def mesh_set_bench(a):
+    """16 meshes of uneven size (displaced tori, 1e4..1e7 triangles, log-uniform)
+    and 64 orbit cameras at resolutions drawn from {256x256, 640x480, 1280x720,
+    1920x1080}; camera i shows mesh i mod 16 (single-object frames), edges +
+    matcap.  Frames render one after another (multi-frame batching is switched
+    off).  A sweep over BVH width {4, 8} x rays per work item {1, 2, 4} x LDS
+    stack depth {8, 16, 32} times all 64 frames per configuration and keeps the
+    fastest.  Parity: the width-4 and width-8 G-buffers of every frame must be
+    identical, and a 64 x 64 crop of every frame must match the oracle."""
+    import ctypes as C
+    import j3d_amd as J
+    from j3d_amd import ffi
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from scenes import compare_gbuffers
+    lib = ffi.load()
+    t0 = time.time()
+    meshes = mesh_set(16, 41)
+    n_tris = [len(m[3]) for m in meshes]
+    log("16 meshes, %d triangles (%d .. %d) in %.1fs" % (sum(n_tris), min(n_tris), max(n_tris), time.time() - t0))
+    r = J.Renderer(0)
+    mc, cav = J.make_matcap(0)
+    r.set_matcap(mc, cav)
+    t0 = time.time()
+    slots = {4: [], 8: []}
+    framing = []
+    for i, (nu, nv, v, t) in enumerate(meshes):
+        for w in (4, 8):
+            slots[w].append(r.upload_mesh(v, t, db_id=J.make_db_id(1, i), width=w))
+        framing.append(J.scene_framing(v))
+    log("uploads (both widths): %.1fs" % (time.time() - t0))
+    s = C.c_uint64(43)
+    sizes = [(256, 256), (640, 480), (1280, 720), (1920, 1080)]
+    cam = J.camera_with_vertical_fov(60.0)
+    cams = []
+    for i in range(64):
+        az = 2.0 * math.pi * lib.j3dg_uniform01(C.byref(s))
+        el = math.radians(-30.0 + 90.0 * lib.j3dg_uniform01(C.byref(s)))
+        dist = 2.6 + lib.j3dg_uniform01(C.byref(s))
+        wh = sizes[min(3, int(4 * lib.j3dg_uniform01(C.byref(s))))]
+        cs, ci = J.orbit_camera(dist, az, el)
+        cams.append((cs, ci, wh))
+    flags = ffi.EDGES | ffi.SHADING
+    frames = {w: [] for w in (4, 8)}
+    for i, (cs, ci, (fw, fh)) in enumerate(cams):
+        diag, pivot = framing[i % 16]
+        for w in (4, 8):
+            frames[w].append(J.make_frame(fw, fh, cam, cs, ci, pivot, diag, [slots[w][i % 16]], flags=flags))
+    targets = [r.create_target(fw, fh) for _, _, (fw, fh) in cams]
+
+    def pass_ms(frs):
+        r.synchronize()
+        t1 = time.perf_counter()
+        for f, tg in zip(frs, targets):
+            r.render(f, tg, readback=False)
+        r.synchronize()
+        return (time.perf_counter() - t1) * 1e3
+
+    base = r.get_tuning()
+    sweep = []
+    for w in (4, 8):
+        for depth in (8, 16, 32):
+            for rpi in (1, 2, 4):
+                r.set_tuning(**dict(base, stack_depth=depth, rays_per_item=rpi))
+                pass_ms(frames[w])
+                ms = min(pass_ms(frames[w]) for _ in range(2))
+                sweep.append({"width": w, "stack_depth": depth, "rays_per_item": rpi, "ms": ms})
+    best = min(sweep, key=lambda e: e["ms"])
+    log("sweep best: %s (worst %.2f ms)" % (best, max(e["ms"] for e in sweep)))
+    r.set_tuning(**dict(base, stack_depth=best["stack_depth"], rays_per_item=best["rays_per_item"]))
+    bf = frames[best["width"]]
+
+    # rays per pass, and the G-buffers of both widths (must agree bit for bit)
+    r.set_counting(True)
+    rays, hits, nodes, tris = 0, 0, 0, 0
+    outs = {4: [], 8: []}
+    for w in (4, 8):
+        for f, tg in zip(frames[w], targets):
+            g, im = r.render(f, tg)
+            outs[w].append((g, im))
+            if w == best["width"]:
+                st = r.stats()
+                rays += st["primary_rays"] + st["shadow_rays"]
+                hits += st["hits"]
+                nodes += st["node_visits"]
+                tris += st["tri_tests"]
+    r.set_counting(False)
+    width_mism = 0
+    for (g4, i4), (g8, i8) in zip(outs[4], outs[8]):
+        width_mism += int(sum(np.count_nonzero(g4[k].view(np.uint32) != g8[k].view(np.uint32))
+                              if g4[k].dtype.itemsize == 4 else np.count_nonzero(g4[k] != g8[k])
+                              for k in g4.dtype.names)) + int(np.count_nonzero(i4 != i8))
+
+    for _ in range(a.warmup):
+        pass_ms(bf)
+    r.reset_timings()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        for f, tg in zip(bf, targets):
+            r.render(f, tg, readback=False)
+    r.synchronize()
+    wall = time.perf_counter() - t1
+    tm = r.timings()
+    value = rays * a.steps / wall / 1e6
+    log("set: %.1f Mrays/s, %.2f ms per pass of 64 frames" % (value, wall / a.steps * 1e3))
+    node_bytes = 96 if best["width"] == 8 else 64
+    algo = nodes * node_bytes + tris * 48 + 32 * sum(fw * fh for _, _, (fw, fh) in cams)
+    trace_total_ms = float(np.sum(tm["trace_ms"])) / a.steps
+    achieved = algo / (trace_total_ms * 1e-3)
+
+    # oracle parity on a 64 x 64 crop of every frame
+    import oracle as O
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    t0 = time.time()
+    om = [O.Mesh(v, t, db_id=J.make_db_id(1, i), threads=threads) for i, (_, _, v, t) in enumerate(meshes)]
+    bvh_s = time.time() - t0
+    crop_mism, cpu_s, cpu_rays = {}, 0.0, 0
+    for i, (f, (g, im)) in enumerate(zip(bf, outs[best["width"]])):
+        fw, fh = f.width, f.height
+        x0, y0 = fw // 2 - 32, fh // 2 - 32
+        rect = (x0, y0, x0 + 63, y0 + 63)
+        cf = J.make_frame(fw, fh, cam, cams[i][0], cams[i][1], framing[i % 16][1], framing[i % 16][0],
+                          [slots[best["width"]][i % 16]], flags=flags, rect=rect)
+        og = np.zeros((fh, fw), O.PIXEL_DTYPE)
+        og["object_id"] = 0xFFFFFFFF
+        t1 = time.time()
+        og, ost = O.update_canvas([om[i % 16]], cf, gbuf=og, threads=threads)
+        cpu_s += time.time() - t1
+        cpu_rays += ost["primary_rays"] + ost["shadow_rays"]
+        for k, n in compare_gbuffers(g[y0:y0 + 64, x0:x0 + 64], og[y0:y0 + 64, x0:x0 + 64]).items():
+            crop_mism[k] = crop_mism.get(k, 0) + n
+    log("oracle crops: %s" % {k: n for k, n in crop_mism.items() if n})
+    desc = {"workload": "16 displaced tori, %d triangles in all (%d .. %d, log-uniform from 1e4..1e7); 64 orbit "
+                        "cameras at 256x256 / 640x480 / 1280x720 / 1920x1080, camera i -> mesh i mod 16; edges + "
+                        "matcap" % (sum(n_tris), min(n_tris), max(n_tris)),
+            "meshes": [[int(nu), int(nv), int(len(t))] for nu, nv, _, t in meshes],
+            "resolutions": [list(c[2]) for c in cams],
+            "launches": "one persistent k_trace + k_shade per frame (multi-frame batching is switched off)"}
+    out = {"metric": "Mrays/s, 16 meshes x 64 cameras of mixed size, sequential frames", "value": value,
+           "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic", "config": desc,
+           "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK, "traffic": None,
+                        "kernel": "k_trace (64 launches per step, summed)", "launch_ms": trace_total_ms,
+                        "algorithmic_bytes_per_launch": int(algo),
+                        "per_unit": "%d B per node visit, 48 B per triangle test, 32 B G-buffer write per pixel"
+                                    % node_bytes},
+           "rays_per_step": int(rays), "hit_pixels": int(hits), "sweep": sweep, "best": best,
+           "width4_vs_width8_mismatches": width_mism,
+           "parity": {"crop": "64x64 at every frame's centre", "gbuffer_mismatches": crop_mism},
+           "cpu_baseline": {"value": cpu_rays / cpu_s / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+                            "sample": "64 crops of 64x64 pixels (%d rays), oracle update_canvas, %.2fs (oracle BVH "
+                                      "builds %.1fs excluded)" % (cpu_rays, cpu_s, bvh_s)}}
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -405,6 +581,8 @@ def main():
         sys.exit(2)
     if a.workload == "torus300m":
         return multi_camera(a)
+    if a.workload == "set16x64":
+        return mesh_set_bench(a)
     import j3d_amd as J
     from j3d_amd import ffi
     v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc = workload(a.workload)
