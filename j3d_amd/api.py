@@ -341,7 +341,7 @@ class Renderer:
     def synchronize(self):
         self._c(self.lib.j3dg_synchronize(self.ctx), "j3dg_synchronize")
 
-    def set_tuning(self, stack_depth=16, rays_per_item=1, coop_tail=1, block_size=256, refill_min=8, split_passes=0,
+    def set_tuning(self, stack_depth=16, rays_per_item=1, coop_tail=1, block_size=256, refill_min=32, split_passes=0,
                    cost_order=1):
         t = ffi.Tuning(stack_depth, rays_per_item, coop_tail, block_size, refill_min, split_passes, cost_order)
         self._c(self.lib.j3dg_set_tuning(self.ctx, C.byref(t)), "j3dg_set_tuning")
@@ -403,8 +403,9 @@ class Renderer:
         return out
 
     def autotune(self, frame, target, repeats=5, cache=None, key=None):
-        """Autotunes stack depth x rays per item (and the margin of a learned
-        frame) on `frame` (j3dg_autotune) and applies the result.  With `cache`
+        """Autotunes stack depth x rays per item, then the refill threshold (and
+        the margin of a learned frame) on `frame` (j3dg_autotune) and applies
+        the result.  With `cache`
         (a JSON file path) results are stored and reused under `key`, e.g.
         "<triangles>@<width>x<height>"."""
         import json

@@ -3,8 +3,9 @@
 // For one representative frame it times the trace stages (primary trace,
 // proposal network, finish + shadow trace; HIP events, median of `repeats`
 // frames after one warm-up) over LDS stack depth {8, 16, 32} x rays per
-// work item {1, 2, 4}, keeps the fastest, and for J3DG_LEARNED frames then
-// sweeps the bound margin.  BVH width is chosen by the caller, which owns the
+// work item {1, 2, 4}, keeps the fastest, then tries the refill threshold
+// {8, 16, 32, 48} there, and for J3DG_LEARNED frames then sweeps the bound
+// margin.  BVH width is chosen by the caller, which owns the
 // mesh uploads (one slot per width); the result cache lives with the caller.
 #include <algorithm>
 #include <cstring>
@@ -60,6 +61,21 @@ extern "C" int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32
       rep->n_tried++;
       if (ms < best_ms) best_ms = ms, best = t;
     }
+  const uint32_t refills[4] = {8, 16, 32, 48};
+  const j3dg_tuning at = best;
+  for (uint32_t f : refills) {
+    if (f == at.refill_min) continue;
+    j3dg_tuning t = at;
+    t.refill_min = f;
+    if ((rc = j3dg_set_tuning(ctx, &t))) return rc;
+    float ms = 0.f;
+    if ((rc = time_frame(ctx, *frame, target, repeats, &ms))) {
+      j3dg_set_tuning(ctx, &base);
+      return rc;
+    }
+    rep->n_tried++;
+    if (ms < best_ms) best_ms = ms, best = t;
+  }
   if ((rc = j3dg_set_tuning(ctx, &best))) return rc;
   rep->best = best;
   rep->best_ms = best_ms;

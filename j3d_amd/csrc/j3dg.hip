@@ -1107,7 +1107,7 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
   if (n.stack_depth == 0) n.stack_depth = 16;
   if (n.rays_per_item == 0) n.rays_per_item = 1;
   if (n.block_size == 0) n.block_size = 256;
-  if (n.refill_min == 0) n.refill_min = 8;
+  if (n.refill_min == 0) n.refill_min = 32;
   if (n.refill_min > 64) return ctx->fail(J3DG_ERR_INVALID_ARG, "refill_min must be in [1, 64]");
   if (n.stack_depth != 8 && n.stack_depth != 16 && n.stack_depth != 32)
     return ctx->fail(J3DG_ERR_INVALID_ARG, "stack_depth must be 8, 16 or 32");

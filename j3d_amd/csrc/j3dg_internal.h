@@ -176,7 +176,7 @@ struct Ctx {
   hipEvent_t* cur_ev() { return ring.empty() ? ev : &ring[8 * (ring_n % kRing)]; }
   bool traced = false, shaded = false, mlp_ran = false;
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 1, 256, 8, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
+  j3dg_tuning tuning{16, 1, 1, 256, 32, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   uint32_t* d_queue = nullptr;

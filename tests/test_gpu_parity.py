@@ -183,3 +183,36 @@ def test_cost_ordered_tiles_repeat_frames(renderer):
     finally:
         renderer.destroy_target(tg)
         renderer.remove_mesh(slot)
+
+
+def test_autotune_keeps_results(renderer, tmp_path):
+    """The autotuner tries 9 (depth, rays per item) points and 3 more refill
+    thresholds, applies a valid configuration, and the frame it tunes still
+    matches the oracle bit for bit; a cached result is reused unchanged."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(300, 120, seed=11)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    fr = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+    mc, cav = J.make_matcap(0)
+    renderer.set_matcap(mc, cav)
+    tg = renderer.create_target(480, 270)
+    base = renderer.get_tuning()
+    cache = str(tmp_path / "tune.json")
+    try:
+        rep = renderer.autotune(fr, tg, repeats=2, cache=cache, key="torus@480x270")
+        assert rep["n_tried"] == 12 and not rep["cached"]
+        tun = renderer.get_tuning()
+        assert tun == rep["tuning"]
+        assert tun["stack_depth"] in (8, 16, 32) and tun["rays_per_item"] in (1, 2, 4)
+        assert tun["refill_min"] in (8, 16, 32, 48)
+        assert rep["best_ms"] <= rep["baseline_ms"]
+        g, im = renderer.render(fr, tg)
+        renderer.set_tuning(**base)
+        again = renderer.autotune(fr, tg, cache=cache, key="torus@480x270")
+        assert again["cached"] and renderer.get_tuning() == rep["tuning"]
+    finally:
+        renderer.set_tuning(**base)
+        renderer.destroy_target(tg)
+        renderer.remove_mesh(slot)
+    og, oim, _ = O.render([O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr, mc, cav)
+    _assert_parity(g, im, og, oim, "autotuned torus 480x270")
