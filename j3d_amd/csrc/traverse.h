@@ -51,9 +51,14 @@ __device__ inline float4 gload4(const float4* p) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Reciprocal of a direction component for the slab tests (|d| clamped to
+// 1e-30 so no slab distance is inf * 0).  Only culling uses it, and culling
+// widens every box by a relative 2^-16 (kSlackUp/kSlackDown), so the hardware
+// reciprocal's 1-ulp error (2^-23 relative, scaling a node's entry and exit
+// distances alike) cannot drop a box the ray enters; no hit depends on it.
 __device__ inline float safe_rcp(float d) {
   float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
-  return 1.f / a;
+  return __builtin_amdgcn_rcpf(a);
 }
 
 // Primary rays come in tiles of 8x8 pixels: ray id = tile * 64 + lane.
@@ -185,7 +190,7 @@ __device__ inline void test4(uint32_t lox, uint32_t hix, uint32_t loy, uint32_t 
 // nearest (kNone if no child is entered).
 template <int W, int D, bool COUNT>
 __device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const Ray& r, float thi, Stack<D>& st,
-                                       Counters& c);
+                                       Counters& c, bool ordered = true);
 
 template <int W, int D, bool COUNT>
 __device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t node, const Ray& r, float thi,
@@ -202,7 +207,7 @@ __device__ inline uint32_t visit_node(const float4* __restrict__ nodes, uint32_t
 // ones far-to-near and returns the nearest (kNone if none is entered).
 template <int W, int D, bool COUNT>
 __device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const Ray& r, float thi, Stack<D>& st,
-                                       Counters& c) {
+                                       Counters& c, bool ordered) {
   if (COUNT) c.nodes++;
   const float tlo = r.t_near * kSlackDown;
   const uint32_t ex = __float_as_uint(v[0].w);
@@ -236,9 +241,12 @@ __device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const 
     ref[a] = ref[bb];               \
     ref[bb] = tr;                   \
   }
+  // Any-hit rays (shadows) need no order: when no lane of the wave wants the
+  // nearest child first, the sorting network is skipped and the entered
+  // children are pushed in slot order.
   if (W == 4) {
     J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(1, 2)
-  } else {
+  } else if (__ballot(ordered)) {
     J3D_CSWAP(0, 2) J3D_CSWAP(1, 3) J3D_CSWAP(4, 6) J3D_CSWAP(5, 7)
     J3D_CSWAP(0, 4) J3D_CSWAP(1, 5) J3D_CSWAP(2, 6) J3D_CSWAP(3, 7)
     J3D_CSWAP(0, 1) J3D_CSWAP(2, 3) J3D_CSWAP(4, 5) J3D_CSWAP(6, 7)
@@ -349,7 +357,7 @@ __device__ inline bool trav_step_fetched(Trav& s, Hit& best, Stack<D>& st, Count
     }
     s.cur = kNone;
   } else {
-    s.cur = decode_node<8, D, COUNT>(v, s.r, s.thi, st, c);
+    s.cur = decode_node<8, D, COUNT>(v, s.r, s.thi, st, c, !s.any);
   }
   return s.cur == kNone ? pop_next<D>(s, st) : false;
 }
