@@ -200,6 +200,16 @@ int j3dg_upload_mesh(j3dg_ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n
 /* scene.cpp:43-52 remove_object */
 int j3dg_remove_mesh(j3dg_ctx* ctx, uint32_t slot);
 int j3dg_set_mesh_cs(j3dg_ctx* ctx, uint32_t slot, const float cs[16]);
+/* A mesh's surface colour, scene_object::p_vertex_colors / p_uv_coordinates /
+   p_texture (scene.h:16-18, set by scene.cpp:17-19), used by update_canvas's
+   textured and vertex-colour branches (canvas.cpp:803-834) under
+   J3DG_TEXTURED / J3DG_VERTEXCOLORS.  vertex_colors: rgb floats per vertex
+   (n_vertices*3) or NULL; uv: (u, v) of each triangle's three corners
+   (n_triangles*6 floats) or NULL; texture: tex_w*tex_h 0xAABBGGRR texels, row
+   by row, required with uv.  NULL arrays remove that colour source (an empty
+   vector in the reference). */
+int j3dg_set_mesh_colors(j3dg_ctx* ctx, uint32_t slot, const float* vertex_colors, const float* uv,
+                         const uint32_t* texture, uint32_t tex_w, uint32_t tex_h);
 int j3dg_get_mesh_info(j3dg_ctx* ctx, uint32_t slot, j3dg_mesh_info* info);
 /* Copies the per-triangle normals the upload computed (n_triangles*3 floats). */
 int j3dg_mesh_normals(j3dg_ctx* ctx, uint32_t slot, float* out);

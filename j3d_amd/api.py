@@ -285,6 +285,17 @@ class Renderer:
         a = np.ascontiguousarray(cs, np.float32)
         self._c(self.lib.j3dg_set_mesh_cs(self.ctx, int(slot), ffi.ptr(a)), "j3dg_set_mesh_cs")
 
+    def set_mesh_colors(self, slot, vertex_colors=None, uv=None, texture=None):
+        """Surface colour of a mesh (j3dg_set_mesh_colors): rgb per vertex
+        (n_vertices x 3 floats), (u, v) per triangle corner (n_triangles x 6
+        floats) with a (h, w) uint32 texture; None removes a source."""
+        vc = None if vertex_colors is None else np.ascontiguousarray(vertex_colors, np.float32).reshape(-1, 3)
+        uvs = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
+        tex = None if texture is None else np.ascontiguousarray(texture, np.uint32)
+        th, tw = (0, 0) if tex is None else tex.shape
+        self._c(self.lib.j3dg_set_mesh_colors(self.ctx, int(slot), ffi.ptr(vc), ffi.ptr(uvs), ffi.ptr(tex), int(tw),
+                                              int(th)), "j3dg_set_mesh_colors")
+
     def mesh_info(self, slot):
         mi = ffi.MeshInfo()
         self._c(self.lib.j3dg_get_mesh_info(self.ctx, int(slot), C.byref(mi)), "j3dg_get_mesh_info")

@@ -38,12 +38,75 @@ enum : uint32_t {
 // Lane phases of k_trace.
 enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u, kFinishPhase = 4u };
 
+// The byte a colour channel becomes, (uint8_t)(c * 255.f) of
+// j3d/canvas.cpp:829-831: truncation toward zero to int32 (NaN and values
+// outside the int32 range give 0), then the low 8 bits.
+__device__ inline uint32_t color_byte(float c) {
+  const float x = c * 255.f;
+  if (!(fabsf(x) < 2147483648.f)) return 0u;
+  return (uint32_t)(int32_t)x & 0xFFu;
+}
+
+// The first G-buffer word of a hit (j3d/canvas.cpp:801-834): mark 0, or mark
+// 2 with r, g, b in bytes 1-3 when the object has a surface colour under the
+// frame's flags (texture first, then vertex colours).  Sets `colored`.
+__device__ inline uint32_t hit_color_word(const DevObject& ob, uint32_t flags, uint32_t tri, float u, float v,
+                                          bool& colored) {
+  const float s0 = 1.f - u - v;
+  colored = false;
+  if ((flags & J3DG_TEXTURED) && ob.uv) {
+    const float* q = ob.uv + 6 * (size_t)tri;
+    float cx = s0 * gload(q) + u * gload(q + 2) + v * gload(q + 4);
+    float cy = s0 * gload(q + 1) + u * gload(q + 3) + v * gload(q + 5);
+    // std::max(std::min(c, 1), 0) exactly, NaN included
+    cx = (1.f < cx) ? 1.f : cx;
+    cx = (cx < 0.f) ? 0.f : cx;
+    cy = (1.f < cy) ? 1.f : cy;
+    cy = (cy < 0.f) ? 0.f : cy;
+    const int w = (int)ob.tex_w, h = (int)ob.tex_h;
+    const float fx = cx * (float)w, fy = cy * (float)h;
+    int x = fx == fx ? (int)fx : -1;  // NaN: below 0, clamped like an x86 conversion
+    int y = fy == fy ? (int)fy : -1;
+    x = x < 0 ? 0 : x >= w ? w - 1 : x;
+    y = y < 0 ? 0 : y >= h ? h - 1 : y;
+    const uint32_t color = gload(ob.tex + (size_t)y * ob.tex_w + x);
+    colored = true;
+    return 2u | (color & 0xFFu) << 8 | ((color >> 8) & 0xFFu) << 16 | ((color >> 16) & 0xFFu) << 24;
+  }
+  if ((flags & J3DG_VERTEXCOLORS) && ob.vcol) {
+    const uint32_t* tr = ob.idx + 3 * (size_t)tri;
+    const float* c0 = ob.vcol + 3 * (size_t)gload(tr);
+    const float* c1 = ob.vcol + 3 * (size_t)gload(tr + 1);
+    const float* c2 = ob.vcol + 3 * (size_t)gload(tr + 2);
+    uint32_t word = 2u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      word |= color_byte(gload(c0 + k) * s0 + u * gload(c1 + k) + v * gload(c2 + k)) << (8 * (k + 1));
+    colored = true;
+    return word;
+  }
+  return 0u;
+}
+
+// Writes the first G-buffer word of a hit: the mark byte alone (r, g, b
+// untouched, j3d/canvas.cpp:801) or mark and colour together.
+__device__ inline void store_hit_word(char* pp, uint32_t word, bool colored) {
+  if (colored)
+    *reinterpret_cast<uint32_t*>(pp) = word;
+  else
+    *reinterpret_cast<uint8_t*>(pp) = 0;
+}
+
 // A primary hit's remaining work (j3d/canvas.cpp:788-801): view-space normal
 // object_cs * (CS^-1 * n_tri), mark 0, db_id; and, for shadows, the world-space
 // hit point from the object's vertices and the direction to the light
 // (canvas.cpp:836-853).  r0..r2 is the triangle's completion record
-// (DevObject::fin): its vertices and normal.  Returns whether a shadow ray
-// follows.
+// (DevObject::fin): its vertices and normal.  A frame with surface colours
+// (canvas.cpp:803-834) gets them from k_color after the trace: here the
+// db_id word then carries the object index for it (the colour code stays out
+// of the traversal loop, where it cost registers and time even untaken).
+// Returns whether a shadow ray follows.
+template <bool COLOR>
 __device__ inline bool finish_hit(const FrameParams& f, const DevObject& ob, const Hit& best, bool writer, char* pp,
                                   const float4& r0, const float4& r1, const float4& r2, f4& pos, f4& ld) {
   f4 n = {r0.w, r1.w, r2.w, 0.f};
@@ -52,14 +115,14 @@ __device__ inline bool finish_hit(const FrameParams& f, const DevObject& ob, con
   float* pf = reinterpret_cast<float*>(pp);
   uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
   if (writer) {
-    *reinterpret_cast<uint8_t*>(pp) = 0;  // mark; r, g, b untouched
+    *reinterpret_cast<uint8_t*>(pp) = 0;  // mark; r, g, b untouched (j3d/canvas.cpp:801)
     pf[1] = n.x;
     pf[2] = n.y;
     pf[3] = best.t;
     pu[4] = best.tri;
     pf[5] = best.u;
     pf[6] = best.v;
-    pu[7] = ob.db_id;
+    pu[7] = COLOR ? best.obj : ob.db_id;
   }
   if (!(f.flags & J3DG_SHADOW)) return false;
   f4 V[3];
@@ -111,7 +174,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
                                                uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
                                                uint32_t batch, uint32_t refill_min, uint32_t coop_tail,
                                                uint64_t* timeline, BatchParams bp) {
-  constexpr bool kFrameLike = MODE == kModeFrame || MODE == kModeBatch;
+  constexpr bool kIsFrame = MODE == kModeFrame || MODE == kModeFrameColor;
+  constexpr bool kFrameLike = kIsFrame || MODE == kModeBatch;
   extern __shared__ uint2 lds_stack[];
   // the object table (at most J3DG_MAX_OBJECTS entries outside batches) is read
   // on every ray start and hit completion: keep the block's copy in LDS
@@ -141,11 +205,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   RayPool pool;
   pool.init();
   uint32_t n_rays = n_rays_q;
-  if (MODE == kModeFrame) n_rays = f.n_tiles * 64u;
+  if (kIsFrame) n_rays = f.n_tiles * 64u;
   if (MODE == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
-  const bool learned = MODE == kModeFrame && (f.flags & J3DG_LEARNED) && f.proposal;
+  const bool learned = kIsFrame && (f.flags & J3DG_LEARNED) && f.proposal;
   const bool any_mode = MODE == kModeAnyRays || MODE == kModeShadow;
-  const bool fused = MODE == kModeBatch || (MODE == kModeFrame && f.fused);
+  const bool fused = MODE == kModeBatch || (kIsFrame && f.fused);
 
   // lane state
   uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain, 3 shadow (one-launch frames)
@@ -171,7 +235,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (!pool.exhausted && (n_idle >= refill_min || n_idle == 64)) {
       // hand new rays to idle lanes; ids outside the rectangle leave a lane idle, so repeat
       while (__ballot(phase == 0) && !pool.exhausted) {
-        uint32_t id = MODE == kModeFrame ? pool.take_banded(phase == 0, queue, batch, f.n_tiles)
+        uint32_t id = kIsFrame ? pool.take_banded(phase == 0, queue, batch, f.n_tiles)
                                          : pool.take(phase == 0, queue, batch, n_rays);
         if (id == kNone) continue;
         float tn, tf;
@@ -192,11 +256,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             lnobj = F.n_obj;
             lobjs = objs + F.obj_base;
           }
-          if (MODE == kModeFrame && f.tile_order) tile = gload(f.tile_order + tile);
+          if (kIsFrame && f.tile_order) tile = gload(f.tile_order + tile);
           cost_slot = tile * 64u + (id & 63u);
           pix_trips = 0;
           if (!tile_pixel(F, tile, id & 63u, px, py)) {
-            if (MODE == kModeFrame && f.pixel_cost) f.pixel_cost[cost_slot] = 0;
+            if (kIsFrame && f.pixel_cost) f.pixel_cost[cost_slot] = 0;
             continue;
           }
           primary_ray(F, px, py, wo, wd);
@@ -281,7 +345,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (COUNT && coop) trips_coop++;
     if (phase != 0) {
       if (COUNT) ray_steps++;
-      if (MODE == kModeFrame) pix_trips++;
+      if (kIsFrame) pix_trips++;
       Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
       bool done;
       if (lnobj == 0) {
@@ -298,7 +362,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           const FrameParams& F = frame_ref<MODE>(f, bp, fi);
           char* pp = reinterpret_cast<char*>(F.gbuf) + (size_t)pix * 32;
           f4 pos, ld;
-          if (finish_hit(F, lobjs[best.obj], best, writer, pp, v[0], v[1], v[2], pos, ld)) {
+          if (finish_hit<MODE == kModeFrameColor>(F, lobjs[best.obj], best, writer, pp, v[0], v[1], v[2], pos, ld)) {
             // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
             ev |= kEvShadowRay;
             phase = kShadowPhase;
@@ -347,7 +411,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           phase = 0;
         } else if (MODE == kModeShadow || phase == kShadowPhase) {
           if (best.found) {
-            if (writer) *(reinterpret_cast<uint8_t*>(frame_ref<MODE>(f, bp, fi).gbuf) + (size_t)pix * 32) = 1;
+            // mark |= 1 (j3d/canvas.cpp:855-856): bit 0 of the pixel's first word, colour bits kept
+            if (writer)
+              atomicOr(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(frame_ref<MODE>(f, bp, fi).gbuf) +
+                                                   (size_t)pix * 32),
+                       1u);
             ev |= kEvShadowed;
           }
           phase = 0;
@@ -391,7 +459,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
     }
-    if (MODE == kModeFrame && f.pixel_cost && phase == 0 && pix_trips && writer) {
+    if (kIsFrame && f.pixel_cost && phase == 0 && pix_trips && writer) {
       f.pixel_cost[cost_slot] = (uint16_t)min(pix_trips, 65535u);  // the pixel is complete
       pix_trips = 0;
     }
@@ -476,7 +544,11 @@ __global__ void __launch_bounds__(256) k_finish(FrameParams f, const DevObject* 
         f4 n = {gload(tnrm), gload(tnrm + 1), gload(tnrm + 2), 0.f};
         n = mat_vec(f.cs_inv, n);
         n = mat_vec(ob.cs, n);
-        *reinterpret_cast<uint8_t*>(pp) = 0;  // mark (j3d/canvas.cpp:801)
+        bool colored = false;
+        const uint32_t word = (f.flags & (J3DG_TEXTURED | J3DG_VERTEXCOLORS))
+                                  ? hit_color_word(ob, f.flags, tri, pf[5], pf[6], colored)
+                                  : 0u;
+        store_hit_word(pp, word, colored);  // mark (j3d/canvas.cpp:801-834)
         pf[1] = n.x;
         pf[2] = n.y;
         pu[7] = ob.db_id;
@@ -521,6 +593,28 @@ __global__ void __launch_bounds__(256) k_finish(FrameParams f, const DevObject* 
     __syncthreads();
   }
   if (threadIdx.x == 0 && total) atomicAdd((unsigned long long*)&counters[2], (unsigned long long)total);
+}
+
+// Surface colours of a one-launch frame's hit pixels (j3d/canvas.cpp:803-834):
+// the trace left the object index in the db_id word; mark becomes mark | 2
+// with r, g, b when the object has a colour (the shadow bit kept), and db_id
+// is written.
+__global__ void __launch_bounds__(256) k_color(FrameParams f, const DevObject* __restrict__ objs) {
+  const uint32_t rw = f.x1 >= f.x0 ? (uint32_t)(f.x1 - f.x0 + 1) : 0u;
+  const uint32_t rh = f.y1 >= f.y0 ? (uint32_t)(f.y1 - f.y0 + 1) : 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rw * rh; i += gridDim.x * blockDim.x) {
+    const uint32_t pix = ((uint32_t)f.y0 + i / rw) * f.w + (uint32_t)f.x0 + i % rw;
+    uint32_t* pu = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(f.gbuf) + (size_t)pix * 32);
+    const uint32_t tri = pu[4];
+    if (tri == kMiss) continue;
+    const uint32_t oi = pu[7] < f.n_obj ? pu[7] : 0u;
+    const DevObject& ob = objs[oi];
+    const float* pf = reinterpret_cast<const float*>(pu);
+    bool colored = false;
+    const uint32_t word = hit_color_word(ob, f.flags, tri, pf[5], pf[6], colored);
+    if (colored) pu[0] = (pu[0] & 1u) | word;
+    pu[7] = ob.db_id;
+  }
 }
 
 __global__ void k_reset(uint32_t* queue, uint64_t* counters, uint32_t n_counters) {
@@ -787,6 +881,7 @@ void* trace_kernel(int mode, int width, int depth, bool count) {
   if (mode == kModeRays) return trace_kernel_mode<kModeRays>(width, depth, false);
   if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, false);
   if (mode == kModeShadow) return trace_kernel_mode<kModeShadow>(width, depth, count);
+  if (mode == kModeFrameColor) return trace_kernel_mode<kModeFrameColor>(width, depth, count);
   return trace_kernel_mode<kModeFrame>(width, depth, count);
 }
 
@@ -831,6 +926,11 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
     std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
     o.db_id = m.db_id;
     o.width = m.info.width;
+    o.vcol = m.d_vcol;
+    o.uv = m.d_uv;
+    o.tex = m.d_tex;
+    o.tex_w = m.tex_w;
+    o.tex_h = m.tex_h;
     o.mlp = m.has_proposal ? m.d_mlp_packed : nullptr;
     std::memcpy(o.mlp_norm, m.mlp_norm, sizeof(o.mlp_norm));
     max_stack = std::max(max_stack, m.max_stack);
@@ -893,7 +993,7 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
   // a batch's frames index their objects in the batch's own table
   const DevObject* objs = mode == kModeBatch ? d_bobjs : d_objs;
   uint64_t* tl = nullptr;
-  if (counting && mode == kModeFrame) {
+  if (counting && (mode == kModeFrame || mode == kModeFrameColor)) {
     uint64_t waves = lanes / 64;
     if (waves * 8 > timeline_capacity) {
       if (d_timeline) hipFree(d_timeline);
@@ -960,8 +1060,15 @@ int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, b
     fp.pixel_cost = tg->pixel_cost;
   }
   HIP_TRY(hipEventRecord(e[0], stream), J3DG_ERR_LAUNCH);
-  int rc = launch_trace_kernel(kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width, d_queue);
+  const bool color = fp.fused && (fp.flags & (J3DG_TEXTURED | J3DG_VERTEXCOLORS));
+  int rc = launch_trace_kernel(color ? kModeFrameColor : kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width,
+                               d_queue);
   if (rc) return rc;
+  if (color) {
+    const uint32_t n = (uint32_t)std::min<uint64_t>(((uint64_t)fp.n_tiles * 64 + 255) / 256, (uint64_t)n_cus * 16);
+    hipLaunchKernelGGL(k_color, dim3(std::max(1u, n)), dim3(256), 0, stream, fp, (const DevObject*)d_objs);
+    HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+  }
   if (order) {
     // tile_cost holds the per-tile bins (bytes) followed by the per-block counts
     const uint32_t nb = (fp.n_tiles + kOrderBlock - 1) / kOrderBlock;
@@ -1349,6 +1456,44 @@ int j3dg_remove_mesh(j3dg_ctx* h, uint32_t slot) {
   return J3DG_OK;
 }
 
+int j3dg_set_mesh_colors(j3dg_ctx* h, uint32_t slot, const float* vertex_colors, const float* uv,
+                         const uint32_t* texture, uint32_t tex_w, uint32_t tex_h) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
+  if (uv && (!texture || tex_w == 0 || tex_h == 0))
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "uv coordinates need a texture of nonzero size");
+  if (!uv && texture) return ctx->fail(J3DG_ERR_INVALID_ARG, "a texture needs uv coordinates");
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  Mesh& m = ctx->meshes[slot];
+  if (m.d_vcol) hipFree(m.d_vcol);
+  if (m.d_uv) hipFree(m.d_uv);
+  if (m.d_tex) hipFree(m.d_tex);
+  m.d_vcol = nullptr, m.d_uv = nullptr, m.d_tex = nullptr;
+  m.tex_w = m.tex_h = 0;
+  ctx->obj_cache.clear();
+  ctx->invalidate_graphs();
+  const size_t cb = 3 * (size_t)m.info.n_vertices * sizeof(float);
+  const size_t ub = 6 * (size_t)m.info.n_triangles * sizeof(float);
+  const size_t xb = (size_t)tex_w * tex_h * sizeof(uint32_t);
+  bool ok = true;
+  if (vertex_colors && cb)
+    ok = hipMalloc(&m.d_vcol, cb) == hipSuccess && hipMemcpy(m.d_vcol, vertex_colors, cb, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok && uv && ub)
+    ok = hipMalloc(&m.d_uv, ub) == hipSuccess && hipMemcpy(m.d_uv, uv, ub, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMalloc(&m.d_tex, xb) == hipSuccess && hipMemcpy(m.d_tex, texture, xb, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    if (m.d_vcol) hipFree(m.d_vcol);
+    if (m.d_uv) hipFree(m.d_uv);
+    if (m.d_tex) hipFree(m.d_tex);
+    m.d_vcol = nullptr, m.d_uv = nullptr, m.d_tex = nullptr;
+    return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device allocation or copy for mesh colours failed");
+  }
+  if (m.d_tex) m.tex_w = tex_w, m.tex_h = tex_h;
+  return J3DG_OK;
+}
+
 int j3dg_set_mesh_cs(j3dg_ctx* h, uint32_t slot, const float cs[16]) {
   Ctx* ctx = reinterpret_cast<Ctx*>(h);
   CHECK_CTX();
@@ -1723,6 +1868,9 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
   if (n == 0) return J3DG_OK;
   if (!frames || !targets) return ctx->fail(J3DG_ERR_INVALID_ARG, "null frame or target list");
   if (n > 65536) return ctx->fail(J3DG_ERR_INVALID_ARG, "more than 65536 frames in one batch");
+  for (uint32_t i = 0; i < n; ++i)  // surface colours come from the single-frame k_color pass
+    if (frames[i].flags & (J3DG_TEXTURED | J3DG_VERTEXCOLORS))
+      return ctx->fail(J3DG_ERR_UNSUPPORTED, "textured / vertex-coloured frames render one at a time");
   if (!ctx->d_matcap) return ctx->fail(J3DG_ERR_INVALID_ARG, "no matcap set");
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   std::vector<j3d::FrameParams> fps(n);
@@ -1758,6 +1906,7 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
       std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
       o.db_id = m.db_id;
       o.width = m.info.width;
+      o.vcol = m.d_vcol, o.uv = m.d_uv, o.tex = m.d_tex, o.tex_w = m.tex_w, o.tex_h = m.tex_h;
       objs.push_back(o);
       max_stack = std::max(max_stack, m.max_stack);
     }

@@ -25,7 +25,10 @@ struct DevObject {
   float inv_cs[16];        // world -> object
   uint32_t db_id;
   uint32_t width;
-  uint32_t pad[2];
+  uint32_t tex_w, tex_h;   // texture size (with `tex`)
+  const float* vcol;       // optional rgb per vertex (nullptr = none)
+  const float* uv;         // optional (u, v) per triangle corner, 6 floats per triangle
+  const uint32_t* tex;     // texture for `uv`, row-major 0xAABBGGRR
   const void* mlp;         // learned proposal: packed network (nullptr = none)
   float mlp_norm[4];       // its input normalisation: box centre (object space), half-diagonal
 };
@@ -52,7 +55,9 @@ struct FrameParams {
   float margin;
 };
 
-enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3, kModeBatch = 4 };
+// kModeFrameColor: kModeFrame for frames with surface colours (the hit's
+// object index goes to the db_id word for k_color).
+enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3, kModeBatch = 4, kModeFrameColor = 5 };
 
 // Many frames in one launch (j3dg_render_batch): per-frame parameters on the
 // device and each frame's first tile in the batch's global tile numbering.
@@ -106,6 +111,11 @@ struct Mesh {
   uint32_t* d_idx = nullptr;
   float* d_normals = nullptr;
   float4* d_fin = nullptr;  // hit completion records (see DevObject::fin)
+  // surface colour (j3d/scene.h:16-18): per-vertex rgb, per-corner uv + texture
+  float* d_vcol = nullptr;
+  float* d_uv = nullptr;
+  uint32_t* d_tex = nullptr;
+  uint32_t tex_w = 0, tex_h = 0;
   std::vector<float> normals_host;
   // learned proposal (see mlp.hip): fp32 parameters, packed bf16 fragments
   float* d_mlp = nullptr;
@@ -119,6 +129,11 @@ struct Mesh {
     if (d_idx) hipFree(d_idx);
     if (d_normals) hipFree(d_normals);
     if (d_fin) hipFree(d_fin);
+    if (d_vcol) hipFree(d_vcol);
+    if (d_uv) hipFree(d_uv);
+    if (d_tex) hipFree(d_tex);
+    d_vcol = nullptr, d_uv = nullptr, d_tex = nullptr;
+    tex_w = tex_h = 0;
     if (d_mlp) hipFree(d_mlp);
     if (d_mlp_packed) hipFree(d_mlp_packed);
     d_nodes = nullptr, d_tris = nullptr, d_verts = nullptr, d_idx = nullptr, d_normals = nullptr, d_mlp = nullptr;

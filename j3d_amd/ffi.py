@@ -133,6 +133,7 @@ SIGNATURES = {
     "j3dg_upload_mesh": (_I, [_P, _U32, _P, _U64, _P, _U64, _P, C.POINTER(BuildOpts), C.POINTER(_U32)]),
     "j3dg_remove_mesh": (_I, [_P, _U32]),
     "j3dg_set_mesh_cs": (_I, [_P, _U32, _P]),
+    "j3dg_set_mesh_colors": (_I, [_P, _U32, _P, _P, _P, _U32, _U32]),
     "j3dg_get_mesh_info": (_I, [_P, _U32, C.POINTER(MeshInfo)]),
     "j3dg_mesh_normals": (_I, [_P, _U32, _P]),
     "j3dg_set_matcap": (_I, [_P, _P, _U32, _U32, _U32]),
@@ -180,15 +181,19 @@ _lib = None
 
 def load(path=None):
     """Loads libj3dg.so (once) and declares every exported function.  The
-    J3DG_LIBRARY environment variable names another build of it."""
+    J3DG_LIBRARY environment variable names another build of it (an older one
+    for A/B timing may lack newer entry points: those stay undeclared)."""
     global _lib
     if _lib is not None:
         return _lib
-    path = path or os.environ.get("J3DG_LIBRARY") or LIB_PATH
+    other = os.environ.get("J3DG_LIBRARY")
+    path = path or other or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError("libj3dg.so is not built (%s); run __graft_entry__.build()" % path)
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
+        if other and not hasattr(lib, name):
+            continue
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args

@@ -830,6 +830,44 @@ void orc_any_hits(const orc_object* objs, uint32_t nobj, const float* rays, uint
   });
 }
 
+uint8_t orc_color_byte(float c) {
+  const float x = c * 255.f;
+  if (!(std::fabs(x) < 2147483648.f)) return 0;
+  return (uint8_t)((uint32_t)(int32_t)x & 0xFFu);
+}
+
+int orc_hit_color(const orc_object* ob, uint32_t flags, uint32_t tri, float u, float v, uint8_t rgb[3]) {
+  const float s0 = 1.f - u - v;
+  if ((flags & ORC_TEXTURED) && ob->uv) {
+    /* j3d/canvas.cpp:803-821: interpolated uv, clamped to [0, 1], nearest texel */
+    const float* uv = ob->uv + 6 * (size_t)tri;
+    float cx = s0 * uv[0] + u * uv[2] + v * uv[4];
+    float cy = s0 * uv[1] + u * uv[3] + v * uv[5];
+    cx = std::max(std::min(cx, 1.f), 0.f);
+    cy = std::max(std::min(cy, 1.f), 0.f);
+    const int w = (int)ob->tex_w, h = (int)ob->tex_h;
+    int x = (int)(cx * (float)w);
+    int y = (int)(cy * (float)h);
+    x = x < 0 ? 0 : x >= w ? w - 1 : x;
+    y = y < 0 ? 0 : y >= h ? h - 1 : y;
+    const uint32_t color = ob->texture[(size_t)y * ob->tex_w + x];
+    rgb[0] = (uint8_t)(color & 0xFFu);
+    rgb[1] = (uint8_t)((color >> 8) & 0xFFu);
+    rgb[2] = (uint8_t)((color >> 16) & 0xFFu);
+    return 1;
+  }
+  if ((flags & ORC_VERTEXCOLORS) && ob->vertex_colors) {
+    /* j3d/canvas.cpp:822-834: barycentric blend of the three vertex colours */
+    const uint32_t* tr = ob->triangles + 3 * (size_t)tri;
+    const float* c0 = ob->vertex_colors + 3 * (size_t)tr[0];
+    const float* c1 = ob->vertex_colors + 3 * (size_t)tr[1];
+    const float* c2 = ob->vertex_colors + 3 * (size_t)tr[2];
+    for (int k = 0; k < 3; ++k) rgb[k] = orc_color_byte(c0[k] * s0 + u * c1[k] + v * c2[k]);
+    return 1;
+  }
+  return 0;
+}
+
 int orc_update_canvas(const orc_object* objs, uint32_t nobj, const orc_frame* f, orc_pixel* gbuf, int threads,
                       int count_ties, orc_stats* st) {
   const uint32_t w = f->w, h = f->h;
@@ -899,6 +937,13 @@ int orc_update_canvas(const orc_object* objs, uint32_t nobj, const orc_frame* f,
         p.barycentric_v = b.v;
         p.db_id = ob.db_id;
         p.mark = 0;
+        uint8_t rgb[3];
+        if (orc_hit_color(&ob, f->flags, b.tri, b.u, b.v, rgb)) {
+          p.r = rgb[0];
+          p.g = rgb[1];
+          p.b = rgb[2];
+          p.mark |= 2;
+        }
         if (shadow) {
           /* j3d/canvas.cpp:836-857 */
           const uint32_t* tr = ob.triangles + 3 * (size_t)b.tri;

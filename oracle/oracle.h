@@ -97,7 +97,20 @@ typedef struct orc_object {
   uint64_t n_triangles;
   float cs[16];            /* object -> world */
   uint32_t db_id;
+  /* optional surface colour (j3d/scene.h:16-18; canvas.cpp:803-834): NULL = none */
+  const float* vertex_colors;  /* rgb float per vertex */
+  const float* uv;             /* per triangle: (u, v) of its three corners */
+  const uint32_t* texture;     /* tex_w x tex_h, row-major, 0xAABBGGRR */
+  uint32_t tex_w, tex_h;
 } orc_object;
+
+/* The byte a colour channel float c becomes: (uint8_t)(c * 255.f) of
+   j3d/canvas.cpp:829-831, defined here as truncation toward zero to int32
+   (NaN and values outside the int32 range give 0), then the low 8 bits. */
+uint8_t orc_color_byte(float c);
+/* Surface colour of a hit (j3d/canvas.cpp:803-834).  Returns 1 and the
+   channels when the object has one under `flags`, else 0. */
+int orc_hit_color(const orc_object* ob, uint32_t flags, uint32_t tri, float u, float v, uint8_t rgb[3]);
 
 typedef struct orc_frame {
   uint32_t w, h;

@@ -34,7 +34,9 @@ class Camera(C.Structure):
 
 class Object(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("vertices", C.c_void_p), ("triangles", C.c_void_p), ("normals", C.c_void_p),
-                ("n_triangles", C.c_uint64), ("cs", C.c_float * 16), ("db_id", C.c_uint32)]
+                ("n_triangles", C.c_uint64), ("cs", C.c_float * 16), ("db_id", C.c_uint32),
+                ("vertex_colors", C.c_void_p), ("uv", C.c_void_p), ("texture", C.c_void_p),
+                ("tex_w", C.c_uint32), ("tex_h", C.c_uint32)]
 
 
 class FrameC(C.Structure):
@@ -84,6 +86,7 @@ def load():
         "orc_closest_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P, _P, _P, _P, _P, _P]),
         "orc_any_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P]),
         "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+        "orc_color_byte": (C.c_uint8, [C.c_float]),
         "orc_mlp_param_count": (C.c_int, []),
         "orc_mlp_line": (None, [_P, C.c_uint64, _P, _P]),
         "orc_mlp_features": (None, [_P, C.c_uint64, _P, _P]),
@@ -159,9 +162,14 @@ def triangle_normals(v, t):
 class Mesh:
     """One scene object for the oracle: arrays, normals and (optionally) a BVH."""
 
-    def __init__(self, verts, tris, cs=None, db_id=(1 << 29), use_bvh=True, threads=0):
+    def __init__(self, verts, tris, cs=None, db_id=(1 << 29), use_bvh=True, threads=0, vertex_colors=None,
+                 uv=None, texture=None):
         self.v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         self.t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        # optional surface colour: rgb per vertex, (u, v) per triangle corner, texture (h, w) uint32
+        self.vcol = None if vertex_colors is None else np.ascontiguousarray(vertex_colors, np.float32).reshape(-1, 3)
+        self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
+        self.tex = None if texture is None else np.ascontiguousarray(texture, np.uint32)
         self.n = triangle_normals(self.v, self.t)
         self.cs = np.eye(4, dtype=np.float32).reshape(16) if cs is None else np.asarray(cs, np.float32)
         self.db_id = int(db_id)
@@ -178,6 +186,11 @@ class Mesh:
         o.n_triangles = self.t.shape[0]
         o.cs[:] = [float(x) for x in self.cs]
         o.db_id = self.db_id
+        o.vertex_colors = None if self.vcol is None else self.vcol.ctypes.data
+        o.uv = None if self.uv is None else self.uv.ctypes.data
+        if self.tex is not None:
+            o.texture = self.tex.ctypes.data
+            o.tex_h, o.tex_w = self.tex.shape
         return o
 
     def __del__(self):
@@ -189,6 +202,11 @@ class Mesh:
 def _objs(meshes, brute=False):
     arr = (Object * max(1, len(meshes)))(*[m.obj(brute) for m in meshes])
     return arr, len(meshes)
+
+
+def color_byte(c):
+    """(uint8_t)(c * 255.f) as the oracle defines it (see oracle.h)."""
+    return int(load().orc_color_byte(C.c_float(float(c))))
 
 
 def update_canvas(meshes, frame, gbuf=None, threads=0, count_ties=False, brute=False):

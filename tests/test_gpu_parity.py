@@ -216,3 +216,53 @@ def test_autotune_keeps_results(renderer, tmp_path):
         renderer.remove_mesh(slot)
     og, oim, _ = O.render([O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr, mc, cav)
     _assert_parity(g, im, og, oim, "autotuned torus 480x270")
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("source", ["vertex", "texture", "both"])
+def test_surface_colors(renderer, source, split):
+    """Textured and vertex-coloured hits (j3d/canvas.cpp:803-834) with shadows:
+    random vertex colours in [-0.1, 1.1] (byte wrap-around included), random
+    per-corner uv in [-0.2, 1.2] (clamping) on a 37x23 texture; colour and
+    shadow marks combine (mark 3).  One launch and the split pipeline."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(200, 90, seed=13)
+    rng = np.random.default_rng(17)
+    vc = rng.uniform(-0.1, 1.1, (len(v), 3)).astype(np.float32)
+    uv = rng.uniform(-0.2, 1.2, (len(t), 6)).astype(np.float32)
+    tex = rng.integers(0, 2 ** 32, (23, 37), dtype=np.uint64).astype(np.uint32)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(3, 2))
+    kw = {"vertex": dict(vertex_colors=vc), "texture": dict(uv=uv, texture=tex),
+          "both": dict(vertex_colors=vc, uv=uv, texture=tex)}[source]
+    renderer.set_mesh_colors(slot, **kw)
+    flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT | ffi.TEXTURED | ffi.VERTEXCOLORS
+    fr = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
+    tun = renderer.get_tuning()
+    renderer.set_tuning(**dict(tun, split_passes=split))
+    try:
+        g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(3, 2), **kw)], fr)
+    finally:
+        renderer.set_tuning(**tun)
+    _assert_parity(g, im, og, oim, "coloured torus (%s) split=%d" % (source, split))
+    hit = g["object_id"] != MISS
+    assert set(np.unique(g["mark"][hit])) == {2, 3}
+    # removing the colours gives the uncoloured frame (mark 0 / 1)
+    renderer.set_mesh_colors(slot)
+    tg = renderer.create_target(480, 270)
+    try:
+        g2, _ = renderer.render(fr, tg)
+    finally:
+        renderer.destroy_target(tg)
+    assert set(np.unique(g2["mark"][g2["object_id"] != MISS])) == {0, 1}
+    renderer.remove_mesh(slot)
+
+
+def test_surface_color_arguments(renderer):
+    v, t = J.torus(20, 10, seed=1)
+    slot = renderer.upload_mesh(v, t)
+    with pytest.raises(J.J3dgError):
+        renderer.set_mesh_colors(slot, uv=np.zeros((len(t), 6), np.float32))  # uv without a texture
+    with pytest.raises(J.J3dgError):
+        renderer.set_mesh_colors(slot, texture=np.zeros((2, 2), np.uint32))  # texture without uv
+    with pytest.raises(J.J3dgError):
+        renderer.set_mesh_colors(99, vertex_colors=np.zeros((len(v), 3), np.float32))
+    renderer.remove_mesh(slot)

@@ -306,6 +306,82 @@ def test_shadow_ray_occluder():
             assert g["mark"][y, x] == 0
 
 
+def test_color_byte_conversion():
+    # (uint8_t)(c * 255.f): truncation toward zero, low 8 bits, NaN / huge -> 0
+    cases = {0.0: 0, 0.5: 127, 1.0: 255, 0.999: 254, 1.2: 50, -0.01: 254, -1.0: 1, float("nan"): 0, 1e10: 0}
+    for c, b in cases.items():
+        assert O.color_byte(c) == b, (c, O.color_byte(c))
+
+
+def _color_frame(flags, w=40, h=30):
+    _, Pi = O.projection(O.default_camera(), w, h)
+    cs = np.eye(4, dtype=np.float32).reshape(16)
+    cs[14] = 3.0
+    return _Frame(w, h, cs, (0, 0, 0), 20.0, flags, Pi)
+
+
+def test_vertex_colors_constant_and_blend():
+    """canvas.cpp:822-834.  A constant colour 0.5 at every vertex gives byte
+    127 everywhere (0.5 scales the barycentric sum exactly, and s0 + u + v is
+    1 within a few ulps); pure red, green and blue vertices give channels that
+    add up to 255 within rounding, each largest at its own vertex."""
+    v, t = _plane()
+    fr = _color_frame(O.VERTEXCOLORS | O.EDGES)
+    g, _ = O.update_canvas([O.Mesh(v, t, vertex_colors=np.full((4, 3), 0.5, np.float32))], fr)
+    assert np.all(g["mark"] == 2) and np.all(g["r"] == 127) and np.all(g["g"] == 127) and np.all(g["b"] == 127)
+    vc = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 1, 0]], np.float32)
+    g0 = np.zeros((30, 40), O.PIXEL_DTYPE)
+    g0["object_id"] = 0xFFFFFFFF
+    g, _ = O.update_canvas([O.Mesh(v, t, vertex_colors=vc)], fr, gbuf=g0)
+    tot = g["r"].astype(int) + g["g"].astype(int) + g["b"].astype(int)
+    assert np.all(tot >= 252) and np.all(tot <= 255)
+    # the camera looks down -z at the centre of the plane: vertex 0 (-x, -y) shows in the bottom-left
+    # quadrant (rows grow downward), vertex 2 (+x, +y) top-right
+    assert g["r"][-1, 0] > g["b"][-1, 0] and g["b"][0, -1] > g["r"][0, -1]
+    # flags off: no colour, r/g/b untouched
+    fr2 = _color_frame(O.EDGES)
+    g1 = np.zeros((30, 40), O.PIXEL_DTYPE)
+    g1["r"] = 9
+    g1, _ = O.update_canvas([O.Mesh(v, t, vertex_colors=vc)], fr2, gbuf=g1)
+    assert np.all(g1["mark"] == 0) and np.all(g1["r"] == 9)
+
+
+def test_texture_lookup_convention_and_priority():
+    """canvas.cpp:803-821.  Every corner of both triangles at one uv point
+    selects one texel (x = int(u w), y = int(v h), texel = texture[y][x]);
+    uv outside [0, 1] clamps to the border texel; with both sources and both
+    flags the texture wins."""
+    v, t = _plane()
+    tex = np.array([[0xff0000aa, 0xff00bb00], [0xffcc0000, 0xff112233]], np.uint32)  # rows y = 0, 1
+    exp = {(0.25, 0.25): (0xaa, 0, 0), (0.75, 0.25): (0, 0xbb, 0), (0.25, 0.75): (0, 0, 0xcc),
+           (0.75, 0.75): (0x33, 0x22, 0x11), (5.0, -3.0): (0, 0xbb, 0), (-1.0, 9.0): (0, 0, 0xcc)}
+    fr = _color_frame(O.TEXTURED | O.VERTEXCOLORS | O.EDGES)
+    for (cu, cv), rgb in exp.items():
+        uv = np.tile(np.array([cu, cv], np.float32), (2, 3))
+        m = O.Mesh(v, t, uv=uv, texture=tex, vertex_colors=np.full((4, 3), 0.5, np.float32))
+        g, _ = O.update_canvas([m], fr)
+        assert np.all(g["mark"] == 2)
+        assert np.all(g["r"] == rgb[0]) and np.all(g["g"] == rgb[1]) and np.all(g["b"] == rgb[2]), (cu, cv)
+
+
+def test_colour_and_shadow_marks_combine():
+    """mark = 2 | 1 on a coloured, shadowed pixel (canvas.cpp:832, 855)."""
+    v1, t1 = _plane(z=0.0, size=2.0)
+    v2, t2 = _plane(z=1.0, size=0.5)
+    v = np.vstack([v1, v2])
+    t = np.vstack([t1, t2 + 4]).astype(np.uint32)
+    w, h = 80, 60
+    _, Pi = O.projection(O.default_camera(), w, h)
+    cs = np.eye(4, dtype=np.float32).reshape(16)
+    cs[14] = 8.0
+    fr = _Frame(w, h, cs, (0, 0, 0), 4.0, O.EDGES | O.SHADOW | O.DIRECTIONAL_LIGHT | O.VERTEXCOLORS, Pi,
+                light=np.array([0.6, 0.0, 0.8], np.float32))
+    g, st = O.update_canvas([O.Mesh(v, t, vertex_colors=np.full((8, 3), 0.25, np.float32))], fr)
+    hit = g["object_id"] != 0xFFFFFFFF
+    assert st["shadowed"] > 0 and set(np.unique(g["mark"][hit])) == {2, 3}
+    assert np.sum(g["mark"] == 3) == st["shadowed"]
+
+
 def test_empty_scene_all_miss():
     w, h = 16, 8
     _, Pi = O.projection(O.default_camera(), w, h)
