@@ -112,6 +112,23 @@ void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const
   s.objects.emplace_back(std::move(obj));
 }
 
+// j3d/scene.h:16-18, scene.cpp:17-19
+void set_object_colors(uint32_t id, scene& s, const std::vector<float>* vertex_colors, const std::vector<float>* uv,
+                       const image<uint32_t>* texture) {
+  auto it = std::find_if(s.objects.begin(), s.objects.end(), [&](const scene_object& so) { return so.db_id == id; });
+  if (it == s.objects.end()) throw std::invalid_argument("set_object_colors: unknown object");
+  const size_t nv = it->p_vertices->size() / 3, nt = it->p_triangles->size() / 3;
+  const bool has_vc = vertex_colors && !vertex_colors->empty();
+  const bool has_uv = uv && !uv->empty();
+  if (has_vc && vertex_colors->size() != 3 * nv) throw std::invalid_argument("set_object_colors: 3 floats per vertex");
+  if (has_uv && (uv->size() != 6 * nt || !texture || texture->width() == 0 || texture->height() == 0))
+    throw std::invalid_argument("set_object_colors: 6 floats per triangle and a texture");
+  s.dev->check(j3dg_set_mesh_colors(s.dev->ctx(), it->slot, has_vc ? vertex_colors->data() : nullptr,
+                                    has_uv ? uv->data() : nullptr, has_uv ? texture->row(0) : nullptr,
+                                    has_uv ? texture->width() : 0, has_uv ? texture->height() : 0),
+               "j3dg_set_mesh_colors");
+}
+
 // j3d/scene.cpp:41-49
 void remove_object(uint32_t id, scene& s) {
   auto it = std::find_if(s.objects.begin(), s.objects.end(), [&](const scene_object& so) { return so.db_id == id; });

@@ -144,6 +144,14 @@ struct scene {
 // box and BVH are computed by the device upload.  bvh_width: 4 or 8.
 void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<uint32_t>& triangles,
                 const float4x4& cs = get_identity(), uint32_t bvh_width = 8);
+// The surface colour sources of object `id`: scene_object::p_vertex_colors,
+// p_uv_coordinates and p_texture (j3d/scene.h:16-18, scene.cpp:17-19), used
+// under canvas_settings::vertexcolors / textured (canvas.cpp:803-834).
+// vertex_colors: rgb per vertex; uv: (u, v) of each triangle's corners (6
+// floats per triangle) with `texture`; null or empty = none.  Borrowed only
+// during the call.
+void set_object_colors(uint32_t id, scene& s, const std::vector<float>* vertex_colors,
+                       const std::vector<float>* uv = nullptr, const image<uint32_t>* texture = nullptr);
 // remove_object (j3d/scene.cpp:41-49)
 void remove_object(uint32_t id, scene& s);
 // prepare_scene / unzoom (j3d/scene.cpp:50-110)
