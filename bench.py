@@ -712,8 +712,8 @@ def main():
            "shadow_node_visits_per_ray": st["shadow_node_visits"] / max(1, st["shadow_rays"]),
            "wave_trips": int(st["wave_trips"] + st["shadow_wave_trips"]),
            "lane_steps": int(st["lane_steps"] + st["shadow_lane_steps"]),
-           "simd_utilisation": (st["lane_steps"] + st["shadow_lane_steps"]) /
-                               max(1, 64 * (st["wave_trips"] + st["shadow_wave_trips"])),
+           "lane_steps_per_wave_trip": (st["lane_steps"] + st["shadow_lane_steps"]) /
+                                       max(1, st["wave_trips"] + st["shadow_wave_trips"]),
            "bvh": {"width": a.width, "leaf_size": a.leaf_size or (2 if a.width == 8 else 4),
                    "nodes": int(info["n_nodes"]),
                    "depth": int(info["max_depth"]),
