@@ -334,6 +334,12 @@ def multi_camera(a):
             "algorithmic_bytes_per_launch": int(algo),
             "per_unit": FUSED_PER_UNIT % 96}
 
+    if a.no_cpu_baseline or a.profile_only:
+        r.close()
+        out = {"metric": "Mrays/s @300M-tri %d cameras %dx%d (profile run: no oracle parity)" % (a.cameras, w, h),
+               "value": value, "plain_mrays": plain_value, "roofline": roof, "learned": learned}
+        print(json.dumps(out), flush=True)
+        return
     # oracle parity on a crop of every camera, timed as the CPU baseline
     import oracle as O
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
