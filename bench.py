@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--width", type=int, default=8, choices=[4, 8])
     ap.add_argument("--leaf-size", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="max triangles per BVH leaf (0: the library default, 2 for 8-wide trees)")
+    ap.add_argument("--builder", default="host", choices=["host", "device"],
+                    help="BVH builder: binned SAH on the host (default) or the linear BVH built on the GPU")
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--coop-tail", type=int, default=-1)
@@ -201,6 +203,9 @@ def pick_crop(gbuf, side):
     return (x0, y0, x0 + side - 1, y0 + side - 1)
 
 
+# j3dg_build_opts.builder values (J3DG_BUILD_HOST, J3DG_BUILD_DEVICE)
+BUILDERS = {"host": 0, "device": 1}
+
 FUSED_PER_UNIT = ("%d B per node visit, 48 B per triangle test (primary and shadow), 32 B G-buffer write per pixel, "
                   "48 B hit completion record per hit")
 
@@ -238,7 +243,7 @@ def multi_camera(a):
     mc, cav = J.make_matcap(0)
     r = J.Renderer(0)
     t0 = time.time()
-    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=8)
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=8, builder=BUILDERS[a.builder])
     info = r.mesh_info(slot)
     log("upload: build %.2fs copy %.2fs, %d nodes, depth %d, %.2f GB on device (%.1fs)" % (
         info["build_seconds"], info["upload_seconds"], info["n_nodes"], info["max_depth"],
@@ -396,8 +401,9 @@ def multi_camera(a):
            "primary_rays": int(sum(s["primary_rays"] for s in pst)),
            "shadow_rays": int(sum(s["shadow_rays"] for s in pst)),
            "trace_ms_per_frame": tm["trace_ms"], "shade_ms_per_frame": tm["shade_ms"],
-           "bvh": {"width": 8, "leaf_size": 2, "nodes": int(info["n_nodes"]), "depth": int(info["max_depth"]),
-                   "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
+           "bvh": {"width": 8, "leaf_size": 2, "builder": a.builder, "nodes": int(info["n_nodes"]),
+                   "depth": int(info["max_depth"]), "build_s": info["build_seconds"],
+                   "upload_s": info["upload_seconds"], "device_gb": info["device_bytes"] / 1e9},
            "tuning": r.get_tuning(), "parity": parity, "cpu_baseline": cpu}
     r.close()
     print(json.dumps(out), flush=True)
@@ -603,7 +609,8 @@ def main():
         tun["coop_tail"] = a.coop_tail
     r.set_tuning(**tun)
     t0 = time.time()
-    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width, leaf_size=a.leaf_size)
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=a.width, leaf_size=a.leaf_size,
+                         builder=BUILDERS[a.builder])
     info = r.mesh_info(slot)
     log("upload: build %.2fs copy %.2fs, %d nodes, depth %d, %.2f GB on device" % (
         info["build_seconds"], info["upload_seconds"], info["n_nodes"], info["max_depth"],
@@ -721,7 +728,7 @@ def main():
            "lane_steps_per_wave_trip": (st["lane_steps"] + st["shadow_lane_steps"]) /
                                        max(1, st["wave_trips"] + st["shadow_wave_trips"]),
            "bvh": {"width": a.width, "leaf_size": a.leaf_size or (2 if a.width == 8 else 4),
-                   "nodes": int(info["n_nodes"]),
+                   "builder": a.builder, "upload_s": info["upload_seconds"], "nodes": int(info["n_nodes"]),
                    "depth": int(info["max_depth"]),
                    "build_s": info["build_seconds"], "device_gb": info["device_bytes"] / 1e9},
            "tuning": r.get_tuning()}

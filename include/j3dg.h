@@ -99,11 +99,18 @@ typedef struct j3dg_frame_desc {
   float margin;                 /* J3DG_LEARNED: bounded pass tmax = t_pred * (1 + margin) */
 } j3dg_frame_desc;
 
+/* BVH builders (j3dg_build_opts.builder). */
+#define J3DG_BUILD_HOST 0   /* binned SAH on the host, re-packed and uploaded (default) */
+#define J3DG_BUILD_DEVICE 1 /* raw triangles uploaded; normals and a linear (Morton-order) BVH
+                               built on the GPU: builds far faster, traverses slower.  Meshes
+                               of fewer than two triangles are built on the host. */
+
 /* BVH build options for j3dg_upload_mesh (0 = default). */
 typedef struct j3dg_build_opts {
   uint32_t width;     /* 4 or 8 (default) children per node */
   uint32_t leaf_size; /* max triangles per leaf (1..4); 0 = 2 for width 8, 4 for width 4 */
   uint32_t threads;   /* host build threads, 0 = hardware_concurrency */
+  uint32_t builder;   /* J3DG_BUILD_HOST or J3DG_BUILD_DEVICE */
 } j3dg_build_opts;
 
 typedef struct j3dg_mesh_info {
@@ -193,8 +200,9 @@ int j3dg_get_timings(j3dg_ctx* ctx, float* trace_ms, float* shade_ms, float* mlp
 /* ---- scene ------------------------------------------------------------ */
 /* Replaces scene.cpp:8-25 add_object for a mesh: computes triangle normals
    (scene.cpp:20), the bounding box (scene.cpp:22) and the BVH (scene.cpp:23,
-   jtk::qbvh), re-packs it for the GPU and uploads it.  Host arrays are only
-   borrowed for the call.  xyz: n_vertices*3 floats; tris: n_triangles*3. */
+   jtk::qbvh), re-packs it for the GPU and uploads it -- or, with
+   J3DG_BUILD_DEVICE, uploads the raw arrays and derives all of it on the
+   GPU.  Host arrays are only borrowed for the call.  xyz: n_vertices*3 floats; tris: n_triangles*3. */
 int j3dg_upload_mesh(j3dg_ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n_vertices, const uint32_t* tris,
                      uint64_t n_triangles, const float cs[16], const j3dg_build_opts* opts, uint32_t* slot);
 /* scene.cpp:43-52 remove_object */

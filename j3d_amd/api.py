@@ -266,10 +266,12 @@ class Renderer:
     def _c(self, rc, what):
         _check(self.lib, self.ctx, rc, what)
 
-    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=8, leaf_size=0, threads=0):
+    def upload_mesh(self, verts, tris, db_id=None, cs=None, width=8, leaf_size=0, threads=0, builder=0):
+        """builder: ffi.BUILD_HOST (binned SAH on the host) or ffi.BUILD_DEVICE
+        (normals and a linear BVH built on the GPU from the raw arrays)."""
         v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
-        o = ffi.BuildOpts(int(width), int(leaf_size), int(threads))
+        o = ffi.BuildOpts(int(width), int(leaf_size), int(threads), int(builder))
         slot = C.c_uint32(0)
         csa = None if cs is None else np.ascontiguousarray(cs, np.float32)
         if db_id is None:

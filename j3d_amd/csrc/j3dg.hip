@@ -1155,6 +1155,72 @@ using j3d::Mesh;
 using j3d::Target;
 
 
+namespace j3d {
+
+// Mesh bookkeeping shared by both builders: pose, bounding box (scene.cpp:22,
+// mesh.cpp:39-58) and a free slot.
+static void init_mesh(Mesh& m, uint32_t db_id, const float* cs, const float* xyz, uint64_t n_vertices, uint64_t n_triangles) {
+  m.live = true;
+  m.db_id = db_id;
+  if (cs)
+    std::memcpy(m.cs, cs, sizeof(m.cs));
+  else
+    for (int i = 0; i < 16; ++i) m.cs[i] = (i % 5 == 0) ? 1.f : 0.f;
+  invert_orthonormal(m.cs, m.inv_cs);
+  m.info.n_vertices = n_vertices;
+  m.info.n_triangles = n_triangles;
+  if (n_vertices) {
+    for (int k = 0; k < 3; ++k) m.info.bb_min[k] = m.info.bb_max[k] = xyz[k];
+    for (uint64_t i = 1; i < n_vertices; ++i)
+      for (int k = 0; k < 3; ++k) {
+        m.info.bb_min[k] = std::min(m.info.bb_min[k], xyz[3 * i + k]);
+        m.info.bb_max[k] = std::max(m.info.bb_max[k], xyz[3 * i + k]);
+      }
+  }
+}
+
+static uint32_t insert_mesh(Ctx* ctx, Mesh&& m) {
+  uint32_t s = (uint32_t)ctx->meshes.size();
+  for (uint32_t i = 0; i < ctx->meshes.size(); ++i)
+    if (!ctx->meshes[i].live) {
+      s = i;
+      break;
+    }
+  if (s == ctx->meshes.size()) ctx->meshes.emplace_back();
+  ctx->meshes[s] = std::move(m);
+  ctx->obj_cache.clear();
+  return s;
+}
+
+// J3DG_BUILD_DEVICE: raw arrays up, normals, completion records and the
+// linear BVH built on the GPU (bvh_gpu.hip); the normals come back for
+// j3dg_mesh_normals.
+static int upload_mesh_device(Ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n_vertices, const uint32_t* tris,
+                       uint64_t n_triangles, const float* cs, const BuildOptions& bo, uint32_t* slot) {
+  auto t0 = std::chrono::steady_clock::now();
+  Mesh m;
+  init_mesh(m, db_id, cs, xyz, n_vertices, n_triangles);
+  double upload_s = 0.0;
+  int rc = device_build_mesh(ctx, m, xyz, n_vertices, tris, n_triangles, bo.width, bo.leaf_size, &upload_s);
+  if (rc == J3DG_OK) {
+    m.normals_host.resize(3 * (size_t)n_triangles);
+    if (hipMemcpy(m.normals_host.data(), m.d_normals, m.normals_host.size() * sizeof(float), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      rc = ctx->fail(J3DG_ERR_LAUNCH, "copy of the device normals failed");
+  }
+  if (rc != J3DG_OK) {
+    m.release();
+    return rc;
+  }
+  const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  m.info.upload_seconds = upload_s;
+  m.info.build_seconds = total - upload_s;
+  *slot = insert_mesh(ctx, std::move(m));
+  return J3DG_OK;
+}
+
+}  // namespace j3d
+
 extern "C" {
 
 int j3dg_abi_version(void) { return J3DG_ABI_VERSION; }
@@ -1346,6 +1412,11 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   // 96-byte fetch, like a node), four for 4-wide ones
   bo.leaf_size = bo.width == 8 ? 2 : 4;
   if (opts && opts->leaf_size) bo.leaf_size = (int)opts->leaf_size;
+  const uint32_t builder = opts ? opts->builder : J3DG_BUILD_HOST;
+  if (builder != J3DG_BUILD_HOST && builder != J3DG_BUILD_DEVICE)
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "unknown BVH builder");
+  if (builder == J3DG_BUILD_DEVICE && n_triangles >= 2)
+    return j3d::upload_mesh_device(ctx, db_id, xyz, n_vertices, tris, n_triangles, cs, bo, slot);
   BuiltBVH bvh;
   const char* err = nullptr;
   if (!build_bvh(xyz, n_vertices, tris, n_triangles, bo, bvh, &err))
@@ -1368,27 +1439,11 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
     for (auto& th : pool) th.join();
   }
   Mesh m;
-  m.live = true;
-  m.db_id = db_id;
-  if (cs)
-    std::memcpy(m.cs, cs, sizeof(m.cs));
-  else
-    for (int i = 0; i < 16; ++i) m.cs[i] = (i % 5 == 0) ? 1.f : 0.f;
-  invert_orthonormal(m.cs, m.inv_cs);
-  m.info.n_vertices = n_vertices;
-  m.info.n_triangles = n_triangles;
+  j3d::init_mesh(m, db_id, cs, xyz, n_vertices, n_triangles);
   m.info.n_nodes = bvh.n_nodes;
   m.info.width = (uint32_t)bvh.width;
   m.info.max_depth = bvh.max_depth;
   m.max_stack = bvh.max_stack + 1;
-  if (n_vertices) {
-    for (int k = 0; k < 3; ++k) m.info.bb_min[k] = m.info.bb_max[k] = xyz[k];
-    for (uint64_t i = 1; i < n_vertices; ++i)
-      for (int k = 0; k < 3; ++k) {
-        m.info.bb_min[k] = std::min(m.info.bb_min[k], xyz[3 * i + k]);
-        m.info.bb_max[k] = std::max(m.info.bb_max[k], xyz[3 * i + k]);
-      }
-  }
   auto t1 = std::chrono::steady_clock::now();
   m.info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
   m.normals_host = std::move(normals);
@@ -1449,16 +1504,7 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   auto t2 = std::chrono::steady_clock::now();
   m.info.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
   m.info.device_bytes = nb + tb + vb + ib + mb + fb;
-  uint32_t s = (uint32_t)ctx->meshes.size();
-  for (uint32_t i = 0; i < ctx->meshes.size(); ++i)
-    if (!ctx->meshes[i].live) {
-      s = i;
-      break;
-    }
-  if (s == ctx->meshes.size()) ctx->meshes.emplace_back();
-  ctx->meshes[s] = std::move(m);
-  ctx->obj_cache.clear();
-  *slot = s;
+  *slot = j3d::insert_mesh(ctx, std::move(m));
   return J3DG_OK;
 }
 

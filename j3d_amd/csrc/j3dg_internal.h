@@ -264,6 +264,13 @@ struct Ctx {
   }
 };
 
+// Device upload and build of a mesh of 2 .. 2^29-1 triangles (bvh_gpu.hip):
+// fills m's device arrays, n_nodes, width, max_depth, device_bytes and
+// max_stack; m.info's bounding box must be set.  upload_s receives the time of
+// the host -> device copies.  On failure the caller releases m.
+int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, const uint32_t* tris,
+                      uint64_t n_triangles, int width, int leaf_size, double* upload_s);
+
 // canvas::update_canvas's frame set-up (j3d/canvas.cpp:679-705) on the host.
 inline int make_frame_params(const j3dg_frame_desc& fd, const Target& tg, FrameParams& fp) {
   std::memset(&fp, 0, sizeof(fp));

@@ -31,6 +31,8 @@ MLP_PARAMS = 13506
 STAGE_TRACE = 1
 STAGE_SHADE = 2
 STAGE_ALL = 3
+BUILD_HOST = 0
+BUILD_DEVICE = 1
 MAX_OBJECTS = 16
 
 # j3d/pixel.h:11-27
@@ -53,7 +55,8 @@ class FrameDesc(C.Structure):
 
 
 class BuildOpts(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("leaf_size", C.c_uint32), ("threads", C.c_uint32)]
+    _fields_ = [("width", C.c_uint32), ("leaf_size", C.c_uint32), ("threads", C.c_uint32),
+                ("builder", C.c_uint32)]
 
 
 class MeshInfo(C.Structure):

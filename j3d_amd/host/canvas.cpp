@@ -105,7 +105,7 @@ void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const
         obj.max_bb[k] = std::max(obj.max_bb[k], vertices[3 * i + k]);
       }
   }
-  j3dg_build_opts bo{bvh_width, 0, 0};
+  j3dg_build_opts bo{bvh_width, 0, 0, J3DG_BUILD_HOST};
   s.dev->check(j3dg_upload_mesh(s.dev->ctx(), id, vertices.data(), nv, triangles.data(), triangles.size() / 3,
                                 cs.data(), &bo, &obj.slot),
                "j3dg_upload_mesh");

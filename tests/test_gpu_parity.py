@@ -108,6 +108,115 @@ def test_leaf_sizes_and_widths(renderer, width, leaf_size):
     renderer.remove_mesh(slot)
 
 
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("leaf_size", [1, 2, 4])
+def test_device_builder(renderer, width, leaf_size):
+    """The BVH, normals and completion records built on the GPU from the raw
+    arrays (BUILD_DEVICE) give the oracle's frame and hits, and the same
+    normals, bounds and stack-safe traversal as the host build."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(300, 130, seed=5)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(2, 1), width=width, leaf_size=leaf_size,
+                                builder=ffi.BUILD_DEVICE)
+    host = renderer.upload_mesh(v, t, width=width, leaf_size=leaf_size)
+    try:
+        n_dev = renderer.mesh_normals(slot, len(t))
+        assert np.array_equal(n_dev.view(np.uint32), renderer.mesh_normals(host, len(t)).view(np.uint32))
+        info, hinfo = renderer.mesh_info(slot), renderer.mesh_info(host)
+        for k in ("n_vertices", "n_triangles", "width", "bb_min", "bb_max"):
+            assert info[k] == hinfo[k], k
+        assert 0 < info["n_nodes"] < len(t) and info["max_depth"] >= 2
+        print("device build W%d leaf %d: %d nodes, depth %d (host %d nodes, depth %d)" % (
+            width, leaf_size, info["n_nodes"], info["max_depth"], hinfo["n_nodes"], hinfo["max_depth"]))
+    finally:
+        renderer.remove_mesh(host)
+    m = O.Mesh(v, t, db_id=J.make_db_id(2, 1))
+    fr = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [slot], flags=EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT,
+                      light_dir=light)
+    g, im, og, oim, st = _render_both(renderer, None, [m], fr)
+    _assert_parity(g, im, og, oim, "torus 78k tris device build W%d leaf %d" % (width, leaf_size))
+    assert st["shadowed"] > 0
+    rays = random_rays(50000, 17, np.zeros(3), 1.4)
+    got = renderer.trace_rays([slot], rays)
+    ref = O.closest_hits([m], rays)
+    for k in ("obj", "tri"):
+        assert np.array_equal(got[k], ref[k]), k
+    for k in ("t", "u", "v"):
+        assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), k
+    assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays))
+    renderer.remove_mesh(slot)
+
+
+def _degenerate_meshes():
+    """Inputs that stress the device build: every Morton code equal (identical
+    triangles: ties go to the lowest triangle id), a flat strip with one empty
+    axis and a 1e4:1 aspect, and meshes of one, two and three triangles (one
+    triangle is built on the host)."""
+    rng = np.random.default_rng(4)
+    same_v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    same_t = np.tile(np.array([[0, 1, 2]], np.uint32), (3000, 1))
+    g = 60
+    ys, xs = np.meshgrid(np.arange(g + 1, dtype=np.float32), np.arange(g + 1, dtype=np.float32), indexing="ij")
+    strip_v = np.stack([xs.ravel() * 1e-2, ys.ravel() * 1e-6, np.zeros(xs.size, np.float32)], 1).astype(np.float32)
+    a = (np.arange(g)[None, :] + (g + 1) * np.arange(g)[:, None]).ravel().astype(np.uint32)
+    strip_t = np.concatenate([np.stack([a, a + 1, a + g + 1], 1), np.stack([a + 1, a + g + 2, a + g + 1], 1)])
+    small_v = rng.uniform(-1, 1, size=(9, 3)).astype(np.float32)
+    return [("identical x3000", same_v, same_t), ("flat strip", strip_v, strip_t.astype(np.uint32)),
+            ("1 tri", small_v, np.array([[0, 1, 2]], np.uint32)),
+            ("2 tris", small_v, np.array([[0, 1, 2], [3, 4, 5]], np.uint32)),
+            ("3 tris", small_v, np.arange(9, dtype=np.uint32).reshape(3, 3))]
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_device_builder_degenerate(renderer, case):
+    label, v, t = _degenerate_meshes()[case]
+    lo, hi = v.min(0), v.max(0)
+    center = (lo + hi) * 0.5
+    radius = float(max(np.linalg.norm(hi - lo), 1e-3))
+    # half the rays aimed at random points of random triangles, half at random points near the mesh
+    rng = np.random.default_rng(31 + case)
+    k = rng.integers(0, len(t), 2000)
+    b = rng.dirichlet(np.ones(3), 2000).astype(np.float32)
+    target = (v[t[k, 0]] * b[:, :1] + v[t[k, 1]] * b[:, 1:2] + v[t[k, 2]] * b[:, 2:3]).astype(np.float32)
+    o = rng.normal(size=(2000, 3))
+    o = target + 2.0 * radius * o / np.linalg.norm(o, axis=1, keepdims=True)
+    aimed = np.zeros((2000, 8), np.float32)
+    aimed[:, 0:3] = o
+    aimed[:, 3:6] = target - o
+    aimed[:, 6] = 1e-3
+    aimed[:, 7] = np.finfo(np.float32).max
+    rays = np.concatenate([aimed, random_rays(2000, 23 + case, center, radius * 0.6)])
+    m = O.Mesh(v, t)
+    ref = O.closest_hits([m], rays)
+    for width in (4, 8):
+        slot = renderer.upload_mesh(v, t, width=width, builder=ffi.BUILD_DEVICE)
+        got = renderer.trace_rays([slot], rays)
+        for k in ("obj", "tri"):
+            assert np.array_equal(got[k], ref[k]), (label, width, k)
+        for k in ("t", "u", "v"):
+            assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), (label, width, k)
+        assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays)), (label, width)
+        renderer.remove_mesh(slot)
+    print("%s: %d of %d rays hit" % (label, int((ref["obj"] != MISS).sum()), len(rays)))
+
+
+def test_mixed_builders_two_objects(renderer):
+    """A host-built and a device-built object in one two-level scene."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(200, 90, seed=2)
+    v2, t2 = J.icosphere(4, seed=3, amplitude=0.05)
+    cs2 = np.eye(4, dtype=np.float32)
+    cs2[3, :3] = (0.4, 0.3, 0.2)  # column-major: translation in the last column
+    cs2 = cs2.reshape(16)
+    s1 = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0))
+    s2 = renderer.upload_mesh(v2, t2, db_id=J.make_db_id(1, 1), cs=cs2, builder=ffi.BUILD_DEVICE)
+    ms = [O.Mesh(v, t, db_id=J.make_db_id(1, 0)), O.Mesh(v2, t2, db_id=J.make_db_id(1, 1), cs=cs2)]
+    fr = J.make_frame(400, 300, cam, cs, ci, pivot, diag, [s1, s2], flags=EDGES | ffi.SHADOW, light_dir=light)
+    g, im, og, oim, st = _render_both(renderer, None, ms, fr)
+    _assert_parity(g, im, og, oim, "torus host + icosphere device")
+    assert len(np.unique(g["db_id"][g["object_id"] != MISS])) == 2
+    renderer.remove_mesh(s1)
+    renderer.remove_mesh(s2)
+
+
 def test_random_rays_closest_and_any(renderer):
     v, t = J.torus(300, 120, seed=3)
     slot = renderer.upload_mesh(v, t)
