@@ -3,7 +3,7 @@
 // data the host upload also derives: triangle normals (scene.cpp:20) and the
 // hit completion records.
 //
-// Linear BVH: triangles sorted by the 30-bit Morton code of their box centre
+// Linear BVH: triangles sorted by the 63-bit Morton code of their box centre
 // in the mesh's bounds; a binary radix tree over the sorted codes (each
 // internal node found independently from the longest common prefixes of its
 // neighbours); node boxes bottom-up (the second child to arrive at a node
@@ -50,18 +50,21 @@ struct Item {
   uint32_t stack;  // traversal stack entries the ancestors may have pushed
 };
 
-__device__ inline uint32_t spread10(uint32_t v) {
-  v = (v * 0x00010001u) & 0xFF0000FFu;
-  v = (v * 0x00000101u) & 0x0F00F00Fu;
-  v = (v * 0x00000011u) & 0xC30C30C3u;
-  v = (v * 0x00000005u) & 0x49249249u;
+// The 21 low bits of v moved to every third bit.
+__device__ inline uint64_t spread21(uint64_t v) {
+  v &= 0x1FFFFFull;
+  v = (v | v << 32) & 0x1F00000000FFFFull;
+  v = (v | v << 16) & 0x1F0000FF0000FFull;
+  v = (v | v << 8) & 0x100F00F00F00F00Full;
+  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
   return v;
 }
 
 // Per triangle: its box (6 floats, lo then hi) and the Morton code of the box
 // centre; the sort values start as the triangle indices.
 __global__ void k_prims(const float* __restrict__ xyz, const uint32_t* __restrict__ tris, uint32_t n, float3 base,
-                        float3 inv_ext, float* __restrict__ box, uint32_t* __restrict__ code,
+                        float3 inv_ext, float* __restrict__ box, uint64_t* __restrict__ code,
                         uint32_t* __restrict__ index) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -79,24 +82,24 @@ __global__ void k_prims(const float* __restrict__ xyz, const uint32_t* __restric
   const float cy = ((lo[1] + hi[1]) * 0.5f - base.y) * inv_ext.y;
   const float cz = ((lo[2] + hi[2]) * 0.5f - base.z) * inv_ext.z;
   auto q = [](float t) {
-    t = t * 1024.f;
-    return t < 0.f ? 0u : (t > 1023.f ? 1023u : (uint32_t)t);
+    t = t * 2097152.f;
+    return t < 0.f ? 0ull : (t > 2097151.f ? 2097151ull : (uint64_t)t);
   };
-  code[i] = spread10(q(cx)) << 2 | spread10(q(cy)) << 1 | spread10(q(cz));
+  code[i] = spread21(q(cx)) << 2 | spread21(q(cy)) << 1 | spread21(q(cz));
   index[i] = i;
 }
 
 // Length of the common prefix of sorted keys i and j (ties broken by
 // position), -1 outside [0, n).
-__device__ inline int delta(const uint32_t* __restrict__ code, uint32_t n, int i, int j) {
+__device__ inline int delta(const uint64_t* __restrict__ code, uint32_t n, int i, int j) {
   if (j < 0 || j >= (int)n) return -1;
-  const uint32_t a = code[i], b = code[j];
-  if (a != b) return __clz(a ^ b);
-  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+  const uint64_t a = code[i], b = code[j];
+  if (a != b) return __clzll(a ^ b);
+  return 64 + __clz((uint32_t)i ^ (uint32_t)j);
 }
 
 // Internal node i of the radix tree over n >= 2 sorted keys.
-__global__ void k_radix_tree(const uint32_t* __restrict__ code, uint32_t n, BNode* __restrict__ nodes,
+__global__ void k_radix_tree(const uint64_t* __restrict__ code, uint32_t n, BNode* __restrict__ nodes,
                              uint32_t* __restrict__ prim_parent) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= (int)n - 1) return;
@@ -412,7 +415,7 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
             hipMalloc(&m.d_normals, 3 * (size_t)n * sizeof(float)) == hipSuccess &&
             hipMalloc(&m.d_fin, fb + 3 * sizeof(float4)) == hipSuccess &&
             hipMalloc(&m.d_tris, tb + 3 * sizeof(float4)) == hipSuccess && alloc(box, 6 * (size_t)n * 4) &&
-            alloc(code, (size_t)n * 4) && alloc(code2, (size_t)n * 4) && alloc(index, (size_t)n * 4) &&
+            alloc(code, (size_t)n * 8) && alloc(code2, (size_t)n * 8) && alloc(index, (size_t)n * 4) &&
             alloc(index2, (size_t)n * 4) && alloc(bnodes, (size_t)(n - 1) * sizeof(BNode)) &&
             alloc(parents, (size_t)n * 4) && alloc(arrivals, (size_t)(n - 1) * 4) &&
             alloc(q0, (size_t)n * sizeof(Item)) && alloc(q1, (size_t)n * sizeof(Item)) && alloc(counters, 64) &&
@@ -435,16 +438,16 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
   float3 inv = make_float3(hi[0] > lo[0] ? 1.f / (hi[0] - lo[0]) : 0.f, hi[1] > lo[1] ? 1.f / (hi[1] - lo[1]) : 0.f,
                            hi[2] > lo[2] ? 1.f / (hi[2] - lo[2]) : 0.f);
   hipLaunchKernelGGL(k_prims, dim3(gb), dim3(256), 0, st, (const float*)m.d_verts, (const uint32_t*)m.d_idx, n, base,
-                     inv, (float*)box.p, (uint32_t*)code.p, (uint32_t*)index.p);
+                     inv, (float*)box.p, (uint64_t*)code.p, (uint32_t*)index.p);
   HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
   // sort (code, index) pairs
-  hipcub::DoubleBuffer<uint32_t> keys((uint32_t*)code.p, (uint32_t*)code2.p);
+  hipcub::DoubleBuffer<uint64_t> keys((uint64_t*)code.p, (uint64_t*)code2.p);
   hipcub::DoubleBuffer<uint32_t> vals((uint32_t*)index.p, (uint32_t*)index2.p);
   size_t sort_bytes = 0;
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, vals, (int)n, 0, 30, st), J3DG_ERR_LAUNCH);
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, vals, (int)n, 0, 63, st), J3DG_ERR_LAUNCH);
   if (!alloc(sortTmp, sort_bytes)) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device sort storage");
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(sortTmp.p, sort_bytes, keys, vals, (int)n, 0, 30, st), J3DG_ERR_LAUNCH);
-  const uint32_t* skeys = keys.Current();
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(sortTmp.p, sort_bytes, keys, vals, (int)n, 0, 63, st), J3DG_ERR_LAUNCH);
+  const uint64_t* skeys = keys.Current();
   const uint32_t* sorted = vals.Current();
   // radix tree and boxes
   BNode* bn = (BNode*)bnodes.p;
@@ -480,8 +483,8 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
     ++levels;
     n_in = host_c[0];
     // each level descends at least one binary level, and the radix tree over
-    // 30-bit codes with index tie-breaks is at most 62 levels deep
-    if (n_in && levels >= 64) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: tree deeper than expected");
+    // 63-bit codes with index tie-breaks is at most 92 levels deep
+    if (n_in && levels >= 96) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: tree deeper than expected");
     std::swap(qin, qout);
   }
   const uint32_t n_wide = host_c[1];
