@@ -6,8 +6,9 @@
 // Linear BVH: triangles sorted by the 63-bit Morton code of their box centre
 // in the mesh's bounds; a binary radix tree over the sorted codes (each
 // internal node found independently from the longest common prefixes of its
-// neighbours); node boxes bottom-up (the second child to arrive at a node
-// builds its box); then, level by level from the root, each binary node is
+// neighbours); node boxes bottom-up, one launch per tree level (the level
+// lists come from a top-down pass; kernel boundaries order the levels, so no
+// thread waits for or fences against another); then, level by level from the root, each binary node is
 // opened into up to W children (the largest-area internal child first, as the
 // host collapser does), and written as the same quantized wide node, leaf
 // encoding and 48-byte triangle records that build_bvh produces.  A subtree
@@ -37,12 +38,11 @@ namespace lbvh {
 // Binary radix tree node.  Child refs: bit 31 set = a single triangle (its
 // sorted position), else an internal node index.
 struct BNode {
-  uint32_t left, right, parent;
+  uint32_t left, right;
   uint32_t first, last;  // sorted positions covered (inclusive)
   float lo[3], hi[3];
 };
 constexpr uint32_t kPrimBit = 0x80000000u;
-constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 
 // Work item of the level-by-level collapse: binary node -> wide node slot.
 struct Item {
@@ -99,8 +99,7 @@ __device__ inline int delta(const uint64_t* __restrict__ code, uint32_t n, int i
 }
 
 // Internal node i of the radix tree over n >= 2 sorted keys.
-__global__ void k_radix_tree(const uint64_t* __restrict__ code, uint32_t n, BNode* __restrict__ nodes,
-                             uint32_t* __restrict__ prim_parent) {
+__global__ void k_radix_tree(const uint64_t* __restrict__ code, uint32_t n, BNode* __restrict__ nodes) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= (int)n - 1) return;
   const int d = delta(code, n, i, i + 1) - delta(code, n, i, i - 1) >= 0 ? 1 : -1;
@@ -122,21 +121,8 @@ __global__ void k_radix_tree(const uint64_t* __restrict__ code, uint32_t n, BNod
   BNode& nd = nodes[i];
   nd.first = (uint32_t)first;
   nd.last = (uint32_t)last;
-  if (first == gamma) {
-    nd.left = kPrimBit | (uint32_t)gamma;
-    prim_parent[gamma] = (uint32_t)i;
-  } else {
-    nd.left = (uint32_t)gamma;
-    nodes[gamma].parent = (uint32_t)i;
-  }
-  if (last == gamma + 1) {
-    nd.right = kPrimBit | (uint32_t)(gamma + 1);
-    prim_parent[gamma + 1] = (uint32_t)i;
-  } else {
-    nd.right = (uint32_t)(gamma + 1);
-    nodes[gamma + 1].parent = (uint32_t)i;
-  }
-  if (i == 0) nd.parent = kNoParent;
+  nd.left = first == gamma ? kPrimBit | (uint32_t)gamma : (uint32_t)gamma;
+  nd.right = last == gamma + 1 ? kPrimBit | (uint32_t)(gamma + 1) : (uint32_t)(gamma + 1);
 }
 
 __device__ inline void child_box(const BNode* nodes, const float* box, const uint32_t* sorted, uint32_t ref,
@@ -150,27 +136,36 @@ __device__ inline void child_box(const BNode* nodes, const float* box, const uin
   }
 }
 
-// Boxes bottom-up: one thread per triangle climbs from its leaf; at each node
-// the first arriving thread stops and the second (whose sibling's box is then
-// complete) writes the node's box and goes on.
-__global__ void k_boxes(BNode* nodes, const float* __restrict__ box, const uint32_t* __restrict__ sorted,
-                        const uint32_t* __restrict__ prim_parent, uint32_t* __restrict__ arrivals, uint32_t n) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  uint32_t node = prim_parent[p];
-  while (node != kNoParent) {
-    __threadfence();
-    if (atomicAdd(&arrivals[node], 1u) == 0u) return;
-    __threadfence();
-    BNode& nd = nodes[node];
-    float l0[3], h0[3], l1[3], h1[3];
-    child_box(nodes, box, sorted, nd.left, l0, h0);
-    child_box(nodes, box, sorted, nd.right, l1, h1);
-    for (int k = 0; k < 3; ++k) {
-      nd.lo[k] = fminf(l0[k], l1[k]);
-      nd.hi[k] = fmaxf(h0[k], h1[k]);
-    }
-    node = nd.parent;
+// Top-down level lists: the internal children of one level's nodes, appended
+// to the next level's list.
+__global__ void k_level_down(const BNode* __restrict__ nodes, const uint32_t* __restrict__ in, uint32_t n_in,
+                             uint32_t* __restrict__ out, uint32_t cap, uint32_t* __restrict__ count) {
+  const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= n_in) return;
+  const BNode& nd = nodes[in[it]];
+  const uint32_t l = nd.left, r = nd.right;
+  const uint32_t k = (l & kPrimBit) ? 0u : 1u;
+  const uint32_t m = k + ((r & kPrimBit) ? 0u : 1u);
+  if (!m) return;
+  const uint32_t at = atomicAdd(count, m);
+  if (at + m > cap) return;  // only a malformed tree; the host sees the count and fails
+  if (k) out[at] = l;
+  if (!(r & kPrimBit)) out[at + k] = r;
+}
+
+// Boxes of one level's nodes from their children (triangles, or nodes of the
+// level below, written by the previous launch).
+__global__ void k_level_boxes(BNode* __restrict__ nodes, const float* __restrict__ box,
+                              const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ list, uint32_t n_in) {
+  const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= n_in) return;
+  BNode& nd = nodes[list[it]];
+  float l0[3], h0[3], l1[3], h1[3];
+  child_box(nodes, box, sorted, nd.left, l0, h0);
+  child_box(nodes, box, sorted, nd.right, l1, h1);
+  for (int k = 0; k < 3; ++k) {
+    nd.lo[k] = fminf(l0[k], l1[k]);
+    nd.hi[k] = fmaxf(h0[k], h1[k]);
   }
 }
 
@@ -406,7 +401,7 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
     ~Tmp() {
       if (p) hipFree(p);
     }
-  } box, code, code2, index, index2, bnodes, parents, arrivals, q0, q1, counters, sortTmp, last, wide;
+  } box, code, code2, index, index2, bnodes, order, q0, q1, counters, sortTmp, last, wide;
   auto alloc = [&](Tmp& t, size_t bytes) { return hipMalloc(&t.p, std::max<size_t>(bytes, 16)) == hipSuccess; };
   const size_t fb = 3 * (size_t)n * sizeof(float4);
   const size_t tb = 3 * (size_t)n * sizeof(float4);
@@ -417,7 +412,7 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
             hipMalloc(&m.d_tris, tb + 3 * sizeof(float4)) == hipSuccess && alloc(box, 6 * (size_t)n * 4) &&
             alloc(code, (size_t)n * 8) && alloc(code2, (size_t)n * 8) && alloc(index, (size_t)n * 4) &&
             alloc(index2, (size_t)n * 4) && alloc(bnodes, (size_t)(n - 1) * sizeof(BNode)) &&
-            alloc(parents, (size_t)n * 4) && alloc(arrivals, (size_t)(n - 1) * 4) &&
+            alloc(order, (size_t)(n - 1) * 4) &&
             alloc(q0, (size_t)n * sizeof(Item)) && alloc(q1, (size_t)n * sizeof(Item)) && alloc(counters, 64) &&
             alloc(last, ((size_t)n + 3) / 4 * 4) && alloc(wide, (size_t)n * words * 4);
   if (!ok) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device allocation for the device BVH build failed");
@@ -451,11 +446,40 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
   const uint32_t* sorted = vals.Current();
   // radix tree and boxes
   BNode* bn = (BNode*)bnodes.p;
-  hipLaunchKernelGGL(k_radix_tree, dim3((n - 1 + 255) / 256), dim3(256), 0, st, skeys, n, bn, (uint32_t*)parents.p);
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
-                     (uint32_t*)arrivals.p, (uint64_t)(n - 1), 0u);
-  hipLaunchKernelGGL(k_boxes, dim3(gb), dim3(256), 0, st, bn, (const float*)box.p, sorted,
-                     (const uint32_t*)parents.p, (uint32_t*)arrivals.p, n);
+  hipLaunchKernelGGL(k_radix_tree, dim3((n - 1 + 255) / 256), dim3(256), 0, st, skeys, n, bn);
+  HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+  // level lists, root first: level L is order[level_at[L] .. level_at[L + 1])
+  uint32_t* ord = (uint32_t*)order.p;
+  uint32_t* cnt = (uint32_t*)counters.p;
+  std::vector<uint32_t> level_at = {0u, 1u};
+  {
+    const uint32_t root = 0u;
+    HIP_TRY(hipMemcpyAsync(ord, &root, sizeof(root), hipMemcpyHostToDevice, st), J3DG_ERR_LAUNCH);
+    for (;;) {
+      const uint32_t a = level_at[level_at.size() - 2], b = level_at.back();
+      if (a == b) break;
+      if (b > n - 1 || level_at.size() > 100)
+        return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: malformed radix tree");
+      HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st), J3DG_ERR_LAUNCH);
+      hipLaunchKernelGGL(k_level_down, dim3((b - a + 255) / 256), dim3(256), 0, st, (const BNode*)bn, ord + a, b - a,
+                         ord + b, n - 1 - b, cnt);
+      HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+      uint32_t c = 0;
+      HIP_TRY(hipMemcpyAsync(&c, cnt, sizeof(c), hipMemcpyDeviceToHost, st), J3DG_ERR_LAUNCH);
+      HIP_TRY(hipStreamSynchronize(st), J3DG_ERR_LAUNCH);
+      if (c > n - 1 - b) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: malformed radix tree");
+      level_at.push_back(b + c);
+    }
+    if (level_at.back() != n - 1) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: malformed radix tree");
+  }
+  // boxes, deepest level first
+  for (size_t L = level_at.size() - 1; L-- > 0;) {
+    const uint32_t a = level_at[L], b = level_at[L + 1];
+    if (a == b) continue;
+    hipLaunchKernelGGL(k_level_boxes, dim3((b - a + 255) / 256), dim3(256), 0, st, bn, (const float*)box.p, sorted,
+                       (const uint32_t*)(ord + a), b - a);
+  }
+  HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
   hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
                      reinterpret_cast<uint32_t*>(last.p), (uint64_t)(n + 3) / 4, 0u);
   HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
@@ -466,7 +490,6 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
   uint32_t n_in = 1, levels = 0;
   Item* qin = (Item*)q0.p;
   Item* qout = (Item*)q1.p;
-  uint32_t* cnt = (uint32_t*)counters.p;
   while (n_in) {
     const uint32_t zero_hdr[4] = {0u, host_c[1], 0u, host_c[3]};
     HIP_TRY(hipMemcpyAsync(cnt, zero_hdr, sizeof(zero_hdr), hipMemcpyHostToDevice, st), J3DG_ERR_LAUNCH);

@@ -1169,13 +1169,44 @@ static void init_mesh(Mesh& m, uint32_t db_id, const float* cs, const float* xyz
   invert_orthonormal(m.cs, m.inv_cs);
   m.info.n_vertices = n_vertices;
   m.info.n_triangles = n_triangles;
-  if (n_vertices) {
-    for (int k = 0; k < 3; ++k) m.info.bb_min[k] = m.info.bb_max[k] = xyz[k];
-    for (uint64_t i = 1; i < n_vertices; ++i)
+  if (!n_vertices) return;
+  // std::min / std::max folded over the vertices from the first one.  Threads
+  // fold chunks from +-inf and are combined in chunk order: the same result
+  // for every input (a NaN is skipped by both folds, and the first of equal
+  // values is kept), except a NaN in the first vertex, which poisons its axis
+  // in the sequential fold and is applied explicitly.
+  const uint64_t per = 1ull << 20;
+  const uint64_t chunks = (n_vertices + per - 1) / per;
+  std::vector<float> part(6 * chunks);
+  auto fold = [&](uint64_t c) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const uint64_t a = c * per, b = std::min(n_vertices, a + per);
+    for (uint64_t i = a; i < b; ++i)
       for (int k = 0; k < 3; ++k) {
-        m.info.bb_min[k] = std::min(m.info.bb_min[k], xyz[3 * i + k]);
-        m.info.bb_max[k] = std::max(m.info.bb_max[k], xyz[3 * i + k]);
+        lo[k] = std::min(lo[k], xyz[3 * i + k]);
+        hi[k] = std::max(hi[k], xyz[3 * i + k]);
       }
+    for (int k = 0; k < 3; ++k) part[6 * c + k] = lo[k], part[6 * c + 3 + k] = hi[k];
+  };
+  const uint64_t threads = std::min<uint64_t>(chunks, std::max(1u, std::thread::hardware_concurrency()));
+  if (threads <= 1) {
+    for (uint64_t c = 0; c < chunks; ++c) fold(c);
+  } else {
+    std::vector<std::thread> pool;
+    for (uint64_t t = 0; t < threads; ++t)
+      pool.emplace_back([&, t]() {
+        for (uint64_t c = t; c < chunks; c += threads) fold(c);
+      });
+    for (auto& th : pool) th.join();
+  }
+  for (int k = 0; k < 3; ++k) {
+    float lo = xyz[k], hi = xyz[k];
+    for (uint64_t c = 0; c < chunks; ++c) {
+      lo = std::min(lo, part[6 * c + k]);
+      hi = std::max(hi, part[6 * c + 3 + k]);
+    }
+    m.info.bb_min[k] = lo;
+    m.info.bb_max[k] = hi;
   }
 }
 
@@ -1193,8 +1224,8 @@ static uint32_t insert_mesh(Ctx* ctx, Mesh&& m) {
 }
 
 // J3DG_BUILD_DEVICE: raw arrays up, normals, completion records and the
-// linear BVH built on the GPU (bvh_gpu.hip); the normals come back for
-// j3dg_mesh_normals.
+// linear BVH built on the GPU (bvh_gpu.hip); the normals stay on the device
+// (j3dg_mesh_normals reads them back).
 static int upload_mesh_device(Ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n_vertices, const uint32_t* tris,
                        uint64_t n_triangles, const float* cs, const BuildOptions& bo, uint32_t* slot) {
   auto t0 = std::chrono::steady_clock::now();
@@ -1202,12 +1233,6 @@ static int upload_mesh_device(Ctx* ctx, uint32_t db_id, const float* xyz, uint64
   init_mesh(m, db_id, cs, xyz, n_vertices, n_triangles);
   double upload_s = 0.0;
   int rc = device_build_mesh(ctx, m, xyz, n_vertices, tris, n_triangles, bo.width, bo.leaf_size, &upload_s);
-  if (rc == J3DG_OK) {
-    m.normals_host.resize(3 * (size_t)n_triangles);
-    if (hipMemcpy(m.normals_host.data(), m.d_normals, m.normals_host.size() * sizeof(float), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-      rc = ctx->fail(J3DG_ERR_LAUNCH, "copy of the device normals failed");
-  }
   if (rc != J3DG_OK) {
     m.release();
     return rc;
@@ -1581,8 +1606,15 @@ int j3dg_mesh_normals(j3dg_ctx* h, uint32_t slot, float* out) {
   CHECK_CTX();
   if (!out) return ctx->fail(J3DG_ERR_INVALID_ARG, "null output");
   if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
-  const auto& n = ctx->meshes[slot].normals_host;
-  if (!n.empty()) std::memcpy(out, n.data(), n.size() * sizeof(float));
+  const Mesh& m = ctx->meshes[slot];
+  const auto& n = m.normals_host;
+  if (!n.empty()) {
+    std::memcpy(out, n.data(), n.size() * sizeof(float));
+  } else if (m.info.n_triangles && m.d_normals) {
+    HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+    HIP_CALL(hipMemcpy(out, m.d_normals, 3 * m.info.n_triangles * sizeof(float), hipMemcpyDeviceToHost),
+             J3DG_ERR_LAUNCH);
+  }
   return J3DG_OK;
 }
 
