@@ -97,6 +97,10 @@ typedef struct j3dg_frame_desc {
   uint32_t n_objects;           /* scene objects in scene order (two-level index order) */
   uint32_t objects[J3DG_MAX_OBJECTS]; /* mesh slots from j3dg_upload_mesh */
   float margin;                 /* J3DG_LEARNED: bounded pass tmax = t_pred * (1 + margin) */
+  const uint32_t* object_list;  /* nullable: the n_objects mesh slots, any number up to 65536 (scene.h:49
+                                   holds any number of objects); replaces objects[].  Scenes of more than
+                                   J3DG_MAX_OBJECTS objects are traversed through a top-level BVH over the
+                                   objects (canvas.cpp:762); learned frames take at most J3DG_MAX_OBJECTS. */
 } j3dg_frame_desc;
 
 /* BVH builders (j3dg_build_opts.builder). */

@@ -180,11 +180,15 @@ class Frame:
         d.flags = self.flags
         d.light_dir[:] = [float(x) for x in self.light_dir]
         d.bg_top, d.bg_bottom = self.bg_top, self.bg_bottom
-        if len(self.objects) > ffi.MAX_OBJECTS:
-            raise J3dgError("at most %d objects per frame" % ffi.MAX_OBJECTS)
         d.n_objects = len(self.objects)
-        for i, s in enumerate(self.objects):
-            d.objects[i] = int(s)
+        if len(self.objects) > ffi.MAX_OBJECTS:
+            # more objects than objects[] holds: the slots go by object_list (kept alive with the struct)
+            lst = np.ascontiguousarray(self.objects, np.uint32)
+            d.object_list = lst.ctypes.data
+            d._object_list = lst
+        else:
+            for i, s in enumerate(self.objects):
+                d.objects[i] = int(s)
         d.margin = self.margin
         return d
 

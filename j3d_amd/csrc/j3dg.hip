@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -178,12 +179,14 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
                                                uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
                                                uint32_t batch, uint32_t refill_min, uint32_t coop_tail,
                                                uint64_t* timeline, BatchParams bp) {
-  constexpr bool kIsFrame = MODE == kModeFrame || MODE == kModeFrameColor;
-  constexpr bool kFrameLike = kIsFrame || MODE == kModeBatch;
+  constexpr int M = MODE & (kModeTop - 1);
+  constexpr bool kTop = (MODE & kModeTop) != 0;  // more objects than the LDS table: a top-level BVH
+  constexpr bool kIsFrame = M == kModeFrame || M == kModeFrameColor;
+  constexpr bool kFrameLike = kIsFrame || M == kModeBatch;
   extern __shared__ uint2 lds_stack[];
   // the object table (at most J3DG_MAX_OBJECTS entries outside batches) is read
   // on every ray start and hit completion: keep the block's copy in LDS
-  constexpr bool kLdsObjects = MODE != kModeBatch;
+  constexpr bool kLdsObjects = M != kModeBatch && !kTop;
   __shared__ DevObject s_objs[kLdsObjects ? J3DG_MAX_OBJECTS : 1];
   if (kLdsObjects) {
     n_obj = min(n_obj, (uint32_t)J3DG_MAX_OBJECTS);
@@ -194,6 +197,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     __syncthreads();
   }
   const DevObject* const tab = kLdsObjects ? s_objs : objs;
+  const TopNode* const top = kTop ? static_cast<const TopNode*>(f.top_nodes) : nullptr;
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
   Counters c = {0, 0, 0}, cs = {0, 0, 0};  // primary (or query) / shadow-phase work
@@ -210,10 +214,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   pool.init();
   uint32_t n_rays = n_rays_q;
   if (kIsFrame) n_rays = f.n_tiles * 64u;
-  if (MODE == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
+  if (M == kModeShadow) n_rays = __builtin_amdgcn_readfirstlane(*q.count);
   const bool learned = kIsFrame && (f.flags & J3DG_LEARNED) && f.proposal;
-  const bool any_mode = MODE == kModeAnyRays || MODE == kModeShadow;
-  const bool fused = MODE == kModeBatch || (kIsFrame && f.fused);
+  const bool any_mode = M == kModeAnyRays || M == kModeShadow;
+  const bool fused = M == kModeBatch || (kIsFrame && f.fused);
 
   // lane state
   uint32_t phase = 0;  // 0 idle, 1 bounded (learned), 2 plain, 3 shadow (one-launch frames)
@@ -231,6 +235,27 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   best.found = false;
   best.t = best.u = best.v = 0.f;
   best.tri = best.obj = kMiss;
+  // starts the lane's ray (world o, d in wo, wd) over [tn, tf] at its first object
+  auto start_ray = [&](float tn, float tf) {
+    if constexpr (kTop) {
+      s.r.t_near = tn;
+      s.r.t_far = tf;
+      s.tmax = best.found ? fminf(best.t, tf) : tf;
+      s.thi = s.tmax * kSlackUp;
+      s.base = 0;
+      s.cur = 0;  // the top level's root; kNone after this if no object is entered
+      top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
+    } else {
+      begin_object(s, lobjs, 0, wo, wd, tn, tf, best);
+    }
+  };
+  // an object's traversal reported done: unless an any-hit ray has its hit,
+  // the top level supplies the next object
+  auto resume_top = [&](bool done) {
+    if (!kTop || !done || (s.any && best.found)) return done;
+    s.cur = kNone;
+    return top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
+  };
 
   for (;;) {
     uint32_t ev = 0;
@@ -246,7 +271,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         if (kFrameLike) {
           uint32_t px, py;
           uint32_t tile = id >> 6;
-          if (MODE == kModeBatch) {
+          if (M == kModeBatch) {
             uint32_t lo = 0, hi = bp.n_frames;  // first_tile[lo] <= tile < first_tile[hi]
             while (hi - lo > 1) {
               const uint32_t mid = (lo + hi) >> 1;
@@ -255,8 +280,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             fi = lo;
             tile -= gload(bp.first_tile + fi);
           }
-          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
-          if (MODE == kModeBatch) {
+          const FrameParams& F = frame_ref<M>(f, bp, fi);
+          if (M == kModeBatch) {
             lnobj = F.n_obj;
             lobjs = objs + F.obj_base;
           }
@@ -280,7 +305,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             }
           }
           ev |= kEvPrimary;
-        } else if (MODE == kModeShadow) {
+        } else if (M == kModeShadow) {
           const float4* rr = reinterpret_cast<const float4*>(q.rays) + 3 * (size_t)id;
           float4 o4 = gload4(rr), d4 = gload4(rr + 1), p4 = gload4(rr + 2);
           wo = f4{o4.x, o4.y, o4.z, o4.w};
@@ -303,7 +328,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         s.any = any_mode;
         if (COUNT) ray_steps = 0;
         if (lnobj) {
-          begin_object(s, lobjs, 0, wo, wd, tn, tf, best);
+          start_ray(tn, tf);
         } else {
           s.cur = kNone;
           s.r.t_near = tn;
@@ -329,7 +354,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         shfl_from(ray_steps, leader);
         shfl_from(cost_slot, leader);
         shfl_from(pix_trips, leader);
-        if (MODE == kModeBatch) {  // the ray's frame travels with it
+        if (M == kModeBatch) {  // the ray's frame travels with it
           shfl_from(fi, leader);
           shfl_from(lnobj, leader);
           shfl_from(lobjs, leader);
@@ -352,8 +377,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       if (kIsFrame) pix_trips++;
       Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
       bool done;
-      if (lnobj == 0) {
-        done = true;
+      if (lnobj == 0 || (kTop && phase != kFinishPhase && s.cur == kNone)) {
+        done = true;  // no object, or no object box on the ray's way
       } else if (kFrameLike && (phase == kFinishPhase || (W == 8 && !coop))) {
         // one fetch for every lane: a node, a leaf's triangles or a hit's completion record
         const float4* p = phase == kFinishPhase ? lobjs[best.obj].fin + 3 * (size_t)best.tri : step_fetch(s);
@@ -363,10 +388,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         if (phase == kFinishPhase) {
           done = false;
           phase = 0;
-          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
+          const FrameParams& F = frame_ref<M>(f, bp, fi);
           char* pp = reinterpret_cast<char*>(F.gbuf) + (size_t)pix * 32;
           f4 pos, ld;
-          if (finish_hit<MODE == kModeFrameColor>(F, lobjs[best.obj], best, writer, pp, v[0], v[1], v[2], pos, ld)) {
+          if (finish_hit<M == kModeFrameColor>(F, lobjs[best.obj], best, writer, pp, v[0], v[1], v[2], pos, ld)) {
             // the same lane traces the pixel's shadow ray next (any hit, t in [1e-3, inf))
             ev |= kEvShadowRay;
             phase = kShadowPhase;
@@ -376,28 +401,29 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             st.sp = 0;
             s.any = true;
             if (COUNT) ray_steps = 0;
-            begin_object(s, lobjs, 0, wo, wd, 1e-3f, FLT_MAX, best);
+            start_ray(1e-3f, FLT_MAX);
           }
         } else {
           if constexpr (W == 8) {
             // up to kStepsPerTrip traversal steps per loop trip: the trip's
             // fixed work (refills, hit completions, bookkeeping: most of its
             // instructions) is shared by more steps
-            done = trav_step_fetched<D, COUNT>(s, best, st, cc, v);
+            done = resume_top(trav_step_fetched<D, COUNT, kTop>(s, best, st, cc, v));
 #pragma unroll
             for (int extra = 1; extra < kStepsPerTrip; ++extra) {
               if (done) break;
               const float4* p2 = step_fetch(s);
 #pragma unroll
               for (int k = 0; k < 6; ++k) v[k] = gload4(p2 + k);
-              done = trav_step_fetched<D, COUNT>(s, best, st, cc, v);
+              done = resume_top(trav_step_fetched<D, COUNT, kTop>(s, best, st, cc, v));
             }
           }
         }
       } else {
-        done = coop ? coop_step<W, D, COUNT>(s, best, st, cc) : trav_step<W, D, COUNT>(s, best, st, cc);
+        done = resume_top(coop ? coop_step<W, D, COUNT, kTop>(s, best, st, cc)
+                               : trav_step<W, D, COUNT, kTop>(s, best, st, cc));
       }
-      if (done && !(s.any && best.found) && s.obj + 1 < lnobj) {
+      if (!kTop && done && !(s.any && best.found) && s.obj + 1 < lnobj) {
         begin_object(s, lobjs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
         done = false;
@@ -414,10 +440,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         }
       }
       if (done) {
-        if (MODE == kModeAnyRays) {
+        if (M == kModeAnyRays) {
           if (writer) q.any[pix] = best.found ? 1 : 0;
           phase = 0;
-        } else if (MODE == kModeRays) {
+        } else if (M == kModeRays) {
           if (writer) {
             q.obj[pix] = best.found ? best.obj : kMiss;
             q.tri[pix] = best.found ? best.tri : kMiss;
@@ -426,11 +452,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             q.v[pix] = best.found ? best.v : 0.f;
           }
           phase = 0;
-        } else if (MODE == kModeShadow || phase == kShadowPhase) {
+        } else if (M == kModeShadow || phase == kShadowPhase) {
           if (best.found) {
             // mark |= 1 (j3d/canvas.cpp:855-856): bit 0 of the pixel's first word, colour bits kept
             if (writer)
-              atomicOr(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(frame_ref<MODE>(f, bp, fi).gbuf) +
+              atomicOr(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(frame_ref<M>(f, bp, fi).gbuf) +
                                                    (size_t)pix * 32),
                        1u);
             ev |= kEvShadowed;
@@ -445,11 +471,11 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           phase = 2;
           st.sp = 0;
           const float t0 = f.diagonal / 100.f, tb = s.r.t_far;
-          begin_object(s, lobjs, 0, wo, wd, tb >= t0 ? tb : t0, FLT_MAX, best);
+          start_ray(tb >= t0 ? tb : t0, FLT_MAX);
         } else {
           if (phase == 1) ev |= kEvResolved;
           phase = 0;
-          const FrameParams& F = frame_ref<MODE>(f, bp, fi);
+          const FrameParams& F = frame_ref<M>(f, bp, fi);
           char* pp = reinterpret_cast<char*>(F.gbuf) + (size_t)pix * 32;
           float* pf = reinterpret_cast<float*>(pp);
           uint32_t* pu = reinterpret_cast<uint32_t*>(pp);
@@ -481,7 +507,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       pix_trips = 0;
     }
     if (!writer) ev = 0;
-    if ((kFrameLike || MODE == kModeShadow) && __ballot(ev != 0)) {
+    if ((kFrameLike || M == kModeShadow) && __ballot(ev != 0)) {
       acc[0] += (uint32_t)__popcll(__ballot(ev & kEvPrimary));
       acc[1] += (uint32_t)__popcll(__ballot(ev & kEvHit));
       acc[2] += (uint32_t)__popcll(__ballot(ev & kEvShadowed));
@@ -492,23 +518,23 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   }
   // node/triangle/trip/step counters: slots 4, 5, 9, 10 for primary and query
   // launches, 12..15 for the shadow launch
-  const int cb = MODE == kModeShadow ? 12 : 4;
+  const int cb = M == kModeShadow ? 12 : 4;
   if (lane_id() == 0) {
     const int slot[6] = {0, 1, 3, 7, 8, 2};
     for (int k = 0; k < 6; ++k)
       if (acc[k]) atomicAdd((unsigned long long*)&counters[slot[k]], (unsigned long long)acc[k]);
-    if (COUNT) atomicAdd((unsigned long long*)&counters[MODE == kModeShadow ? 14 : 9], (unsigned long long)trips);
+    if (COUNT) atomicAdd((unsigned long long*)&counters[M == kModeShadow ? 14 : 9], (unsigned long long)trips);
   }
   if (COUNT) {
     add_counter(counters, cb, c.nodes);
     add_counter(counters, cb + 1, c.tris);
-    add_counter(counters, MODE == kModeShadow ? 15 : 10, c.steps);
+    add_counter(counters, M == kModeShadow ? 15 : 10, c.steps);
     // per-ray step histogram: slot 16/17 = max steps (primary/shadow), 18.. / 25.. = log2 buckets
-    const int hb = MODE == kModeShadow ? 25 : 18;
+    const int hb = M == kModeShadow ? 25 : 18;
     for (int k = 0; k < 7; ++k) add_counter(counters, hb + k, hist[k]);
     uint32_t mx = max_steps;
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
-    if (lane_id() == 0) atomicMax((unsigned long long*)&counters[MODE == kModeShadow ? 17 : 16], (unsigned long long)mx);
+    if (lane_id() == 0) atomicMax((unsigned long long*)&counters[M == kModeShadow ? 17 : 16], (unsigned long long)mx);
     if (fused) {  // the shadow phase's work goes to the shadow slots
       add_counter(counters, 12, cs.nodes);
       add_counter(counters, 13, cs.tris);
@@ -894,6 +920,10 @@ void* trace_kernel_mode(int width, int depth, bool count) {
 }
 
 void* trace_kernel(int mode, int width, int depth, bool count) {
+  if (mode == (kModeRays | kModeTop)) return trace_kernel_mode<kModeRays | kModeTop>(width, depth, false);
+  if (mode == (kModeAnyRays | kModeTop)) return trace_kernel_mode<kModeAnyRays | kModeTop>(width, depth, false);
+  if (mode == (kModeFrameColor | kModeTop)) return trace_kernel_mode<kModeFrameColor | kModeTop>(width, depth, count);
+  if (mode == (kModeFrame | kModeTop)) return trace_kernel_mode<kModeFrame | kModeTop>(width, depth, count);
   if (mode == kModeBatch) return trace_kernel_mode<kModeBatch>(width, depth, count);
   if (mode == kModeRays) return trace_kernel_mode<kModeRays>(width, depth, false);
   if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, false);
@@ -952,7 +982,144 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
     std::memcpy(o.mlp_norm, m.mlp_norm, sizeof(o.mlp_norm));
     max_stack = std::max(max_stack, m.max_stack);
   }
+  if (n > J3DG_MAX_OBJECTS) {
+    int rc = build_top_level(slots, n);
+    if (rc) return rc;
+    max_stack += top_depth;  // top-level entries sit below an object's
+  }
   return upload_object_table(tab.data(), n);
+}
+
+// World box of a local box under the column-major pose cs, in double, then
+// rounded outward to float.
+static void world_box(const float* cs, const float* lo, const float* hi, float* wlo, float* whi) {
+  double a[3] = {INFINITY, INFINITY, INFINITY}, b[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; ++c) {
+    const double p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+    for (int k = 0; k < 3; ++k) {
+      const double w = (double)cs[k] * p[0] + (double)cs[4 + k] * p[1] + (double)cs[8 + k] * p[2] + (double)cs[12 + k];
+      a[k] = std::min(a[k], w);
+      b[k] = std::max(b[k], w);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    float fl = (float)a[k], fh = (float)b[k];
+    if ((double)fl > a[k]) fl = std::nextafter(fl, -INFINITY);
+    if ((double)fh < b[k]) fh = std::nextafter(fh, INFINITY);
+    wlo[k] = fl;
+    whi[k] = fh;
+  }
+}
+
+// Top level of a scene of more than J3DG_MAX_OBJECTS objects (j3d builds
+// qbvh_two_level_with_transformations over its objects every frame,
+// canvas.cpp:762): a binary BVH over the objects' world boxes, split at the
+// median centroid of the widest axis, one object per leaf.  Rebuilt when an
+// object's box or pose changes.
+int Ctx::build_top_level(const uint32_t* slots, uint32_t n) {
+  if (n > kMaxSceneObjects) return fail(J3DG_ERR_INVALID_ARG, "too many objects in one scene");
+  std::vector<float> key;
+  key.reserve((size_t)n * 22);
+  for (uint32_t i = 0; i < n; ++i) {
+    const Mesh& m = meshes[slots[i]];
+    key.insert(key.end(), m.info.bb_min, m.info.bb_min + 3);
+    key.insert(key.end(), m.info.bb_max, m.info.bb_max + 3);
+    key.insert(key.end(), m.cs, m.cs + 16);
+  }
+  if (d_top && key.size() == top_key.size() &&
+      std::memcmp(key.data(), top_key.data(), key.size() * sizeof(float)) == 0)
+    return J3DG_OK;
+  struct Box {
+    float lo[3], hi[3];
+    float c[3];
+    uint32_t obj;
+  };
+  std::vector<Box> items(n);
+  float radius = 0.f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const Mesh& m = meshes[slots[i]];
+    Box& b = items[i];
+    world_box(m.cs, m.info.bb_min, m.info.bb_max, b.lo, b.hi);
+    for (int k = 0; k < 3; ++k) {
+      b.c[k] = 0.5f * b.lo[k] + 0.5f * b.hi[k];
+      radius = std::max({radius, std::fabs(b.lo[k]), std::fabs(b.hi[k]), std::fabs(m.cs[12 + k]),
+                         std::fabs(m.info.bb_min[k]), std::fabs(m.info.bb_max[k])});
+    }
+    b.obj = i;
+  }
+  if (!std::isfinite(radius)) return fail(J3DG_ERR_INVALID_ARG, "object box or pose is not finite");
+  std::vector<TopNode> nodes;
+  nodes.reserve(std::max<uint32_t>(n, 2));
+  uint32_t depth = 0;
+  auto range_box = [&](uint32_t a, uint32_t b, float* out) {
+    for (int k = 0; k < 3; ++k) out[k] = INFINITY, out[3 + k] = -INFINITY;
+    for (uint32_t i = a; i < b; ++i)
+      for (int k = 0; k < 3; ++k) {
+        out[k] = std::min(out[k], items[i].lo[k]);
+        out[3 + k] = std::max(out[3 + k], items[i].hi[k]);
+      }
+  };
+  // returns the ref of the subtree over items[a, b)
+  std::function<uint32_t(uint32_t, uint32_t, uint32_t)> build = [&](uint32_t a, uint32_t b, uint32_t level) {
+    if (b - a == 1) return kLeafBit | items[a].obj;
+    const uint32_t id = (uint32_t)nodes.size();
+    nodes.emplace_back();
+    depth = std::max(depth, level);
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = a; i < b; ++i)
+      for (int k = 0; k < 3; ++k) clo[k] = std::min(clo[k], items[i].c[k]), chi[k] = std::max(chi[k], items[i].c[k]);
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    const uint32_t mid = a + (b - a) / 2;
+    std::nth_element(items.begin() + a, items.begin() + mid, items.begin() + b, [axis](const Box& x, const Box& y) {
+      return x.c[axis] < y.c[axis] || (x.c[axis] == y.c[axis] && x.obj < y.obj);
+    });
+    const uint32_t l = build(a, mid, level + 1);
+    const uint32_t r = build(mid, b, level + 1);
+    TopNode& nd = nodes[id];
+    range_box(a, mid, nd.box[0]);
+    range_box(mid, b, nd.box[1]);
+    nd.child[0] = l;
+    nd.child[1] = r;
+    nd.pad[0] = nd.pad[1] = 0;
+    return id;
+  };
+  if (n == 1) {
+    TopNode nd;
+    range_box(0, 1, nd.box[0]);
+    for (int k = 0; k < 6; ++k) nd.box[1][k] = 0.f;
+    nd.child[0] = kLeafBit | 0u;
+    nd.child[1] = kNone;
+    nd.pad[0] = nd.pad[1] = 0;
+    nodes.push_back(nd);
+    depth = 1;
+  } else {
+    build(0, n, 1);
+  }
+  Ctx* ctx = this;
+  if (nodes.size() > top_capacity) {
+    HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
+    if (d_top) hipFree(d_top);
+    d_top = nullptr;
+    top_capacity = 0;
+    HIP_TRY(hipMalloc(&d_top, nodes.size() * sizeof(TopNode)), J3DG_ERR_OUT_OF_MEMORY);
+    top_capacity = (uint32_t)nodes.size();
+  }
+  HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
+  HIP_TRY(hipMemcpy(d_top, nodes.data(), nodes.size() * sizeof(TopNode), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+  top_depth = depth;
+  top_radius = radius;
+  top_key = std::move(key);
+  return J3DG_OK;
+}
+
+// A hit found in an object's own frame lies within a few float roundings of
+// the transforms (relative to the coordinates involved: ray origin, object
+// translation and box, distance along the ray) from the world ray; 2^-14 of
+// those magnitudes is a wide margin over that.
+float Ctx::top_delta(float origin_radius) const {
+  return std::ldexp(2.f * (origin_radius + top_radius), -14);
 }
 
 int Ctx::upload_object_table(const DevObject* tab, uint32_t n) {
@@ -1052,7 +1219,9 @@ int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, b
   std::memset(&q, 0, sizeof(q));
   hipEvent_t* e = cur_ev();
   FrameParams fp = fp_in;
-  fp.fused = (tuning.split_passes == 0 && fp.n_obj) ? 1u : 0u;
+  // scenes with a top level always run as one launch (the split pipeline has no top-level variant)
+  const int top = fp.top_nodes ? kModeTop : 0;
+  fp.fused = ((tuning.split_passes == 0 || top) && fp.n_obj) ? 1u : 0u;
   const uint32_t rect_px = fp.n_tiles * 64u;
   const bool order = tg && tuning.cost_order && fp.n_obj && fp.n_tiles;
   if (order) {
@@ -1078,8 +1247,8 @@ int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, b
   }
   HIP_TRY(hipEventRecord(e[0], stream), J3DG_ERR_LAUNCH);
   const bool color = fp.fused && (fp.flags & (J3DG_TEXTURED | J3DG_VERTEXCOLORS));
-  int rc = launch_trace_kernel(color ? kModeFrameColor : kModeFrame, fp, q, rect_px, fp.n_obj, max_stack, width,
-                               d_queue);
+  int rc = launch_trace_kernel((color ? kModeFrameColor : kModeFrame) | top, fp, q, rect_px, fp.n_obj, max_stack,
+                               width, d_queue);
   if (rc) return rc;
   if (color) {
     const uint32_t n = (uint32_t)std::min<uint64_t>(((uint64_t)fp.n_tiles * 64 + 255) / 256, (uint64_t)n_cus * 16);
@@ -1759,7 +1928,13 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
   if (target >= ctx->targets.size() || !ctx->targets[target].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target");
   Target& tg = ctx->targets[target];
   if (fd->width != tg.w || fd->height != tg.h) return ctx->fail(J3DG_ERR_INVALID_ARG, "frame size differs from target size");
-  if (fd->n_objects > J3DG_MAX_OBJECTS) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many objects");
+  // the scene's mesh slots: objects[] (up to J3DG_MAX_OBJECTS), or object_list (any number)
+  const uint32_t* slots = fd->object_list ? fd->object_list : fd->objects;
+  if (!fd->object_list && fd->n_objects > J3DG_MAX_OBJECTS)
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "more than J3DG_MAX_OBJECTS objects need object_list");
+  if (fd->n_objects > j3d::kMaxSceneObjects) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many objects");
+  if ((fd->flags & J3DG_LEARNED) && fd->n_objects > J3DG_MAX_OBJECTS && !(fd->flags & J3DG_NO_SCENE))
+    return ctx->fail(J3DG_ERR_UNSUPPORTED, "learned frames take at most J3DG_MAX_OBJECTS objects");
   if ((stages & J3DG_STAGE_SHADE) && !ctx->d_matcap) return ctx->fail(J3DG_ERR_INVALID_ARG, "no matcap set");
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   ctx->traced = ctx->shaded = ctx->mlp_ran = false;
@@ -1781,12 +1956,18 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
         fp.n_obj = 0;
         width = 4;
       } else {
-        rc = ctx->build_object_table(fd->objects, fd->n_objects, max_stack, width);
+        rc = ctx->build_object_table(slots, fd->n_objects, max_stack, width);
         if (rc) return rc;
+        if (fd->n_objects > J3DG_MAX_OBJECTS) {
+          // primary rays start at the camera, shadow rays at hits inside the scene's boxes
+          const float cam = std::max({std::fabs(fp.cs[12]), std::fabs(fp.cs[13]), std::fabs(fp.cs[14])});
+          fp.top_nodes = ctx->d_top;
+          fp.top_delta = ctx->top_delta(cam);
+        }
       }
       if ((fd->flags & J3DG_LEARNED) && fp.n_obj) {
         bool any_net = false;
-        for (uint32_t i = 0; i < fd->n_objects; ++i) any_net = any_net || ctx->meshes[fd->objects[i]].has_proposal;
+        for (uint32_t i = 0; i < fd->n_objects; ++i) any_net = any_net || ctx->meshes[slots[i]].has_proposal;
         const bool keep = (fd->flags & J3DG_KEEP_PROPOSAL) != 0;
         if (keep || any_net) {
           rc = ensure_proposal(ctx, tg);
@@ -1825,7 +2006,7 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
 static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
                            uint32_t* obj, uint32_t* tri, float* t, float* u, float* v, uint8_t* hit, bool any) {
   if (!rays && n) return ctx->fail(J3DG_ERR_INVALID_ARG, "null rays");
-  if (n_slots > J3DG_MAX_OBJECTS || (n_slots && !slots)) return ctx->fail(J3DG_ERR_INVALID_ARG, "bad object list");
+  if (n_slots > j3d::kMaxSceneObjects || (n_slots && !slots)) return ctx->fail(J3DG_ERR_INVALID_ARG, "bad object list");
   if (n == 0) return J3DG_OK;
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   uint32_t max_stack = 1;
@@ -1865,8 +2046,18 @@ static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
   FrameParams fp;
   std::memset(&fp, 0, sizeof(fp));
   for (int i = 0; i < 16; ++i) fp.cs[i] = (i % 5 == 0) ? 1.f : 0.f;
-  int rc = ctx->launch_trace_kernel(any ? kModeAnyRays : kModeRays, fp, q, (uint32_t)n, n_slots, max_stack, width,
-                                    ctx->d_queue);
+  int mode = any ? kModeAnyRays : kModeRays;
+  if (n_slots > J3DG_MAX_OBJECTS) {
+    float r = 0.f;
+    for (uint64_t i = 0; i < 3 * n; ++i) {
+      const float a = std::fabs(rays[8 * (i / 3) + i % 3]);
+      if (std::isfinite(a)) r = std::max(r, a);  // a ray with a non-finite origin hits nothing
+    }
+    fp.top_nodes = ctx->d_top;
+    fp.top_delta = ctx->top_delta(r);
+    mode |= kModeTop;
+  }
+  int rc = ctx->launch_trace_kernel(mode, fp, q, (uint32_t)n, n_slots, max_stack, width, ctx->d_queue);
   if (rc) return rc;
   HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
@@ -1963,9 +2154,12 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
   if (n == 0) return J3DG_OK;
   if (!frames || !targets) return ctx->fail(J3DG_ERR_INVALID_ARG, "null frame or target list");
   if (n > 65536) return ctx->fail(J3DG_ERR_INVALID_ARG, "more than 65536 frames in one batch");
-  for (uint32_t i = 0; i < n; ++i)  // surface colours come from the single-frame k_color pass
+  for (uint32_t i = 0; i < n; ++i) {  // surface colours come from the single-frame k_color pass
     if (frames[i].flags & (J3DG_TEXTURED | J3DG_VERTEXCOLORS))
       return ctx->fail(J3DG_ERR_UNSUPPORTED, "textured / vertex-coloured frames render one at a time");
+    if (frames[i].object_list)
+      return ctx->fail(J3DG_ERR_UNSUPPORTED, "frames with an object_list render one at a time");
+  }
   if (!ctx->d_matcap) return ctx->fail(J3DG_ERR_INVALID_ARG, "no matcap set");
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   std::vector<j3d::FrameParams> fps(n);

@@ -53,11 +53,19 @@ struct FrameParams {
   j3dg_pixel* gbuf;
   const float2* proposal;   // learned: per-pixel (t bound, hit logit), indexed y * w + x
   float margin;
+  const void* top_nodes;    // kModeTop: the objects' top-level BVH (TopNode[], root first)
+  float top_delta;          // kModeTop: world-box widening for the top level's box tests
 };
 
 // kModeFrameColor: kModeFrame for frames with surface colours (the hit's
 // object index goes to the db_id word for k_color).
 enum { kModeFrame = 0, kModeRays = 1, kModeAnyRays = 2, kModeShadow = 3, kModeBatch = 4, kModeFrameColor = 5 };
+// Added to kModeFrame, kModeFrameColor, kModeRays or kModeAnyRays: scenes of
+// more than J3DG_MAX_OBJECTS objects, with a top-level BVH over the objects
+// (FrameParams::top_nodes) and the object table read from global memory.
+constexpr int kModeTop = 16;
+// Largest object count of one scene with a top level.
+constexpr uint32_t kMaxSceneObjects = 1u << 16;
 
 // Many frames in one launch (j3dg_render_batch): per-frame parameters on the
 // device and each frame's first tile in the batch's global tile numbering.
@@ -201,6 +209,13 @@ struct Ctx {
   DevObject* d_objs = nullptr;
   uint32_t obj_capacity = 0;
   std::vector<uint8_t> obj_cache;
+  // top-level BVH of the last object table of more than J3DG_MAX_OBJECTS
+  // objects (TopNode[]), keyed by the objects' boxes and poses
+  void* d_top = nullptr;
+  uint32_t top_capacity = 0;     // TopNode slots allocated
+  uint32_t top_depth = 0;        // internal levels: stack entries the top level may hold
+  float top_radius = 0.f;        // largest |coordinate| of the world boxes, translations and local boxes
+  std::vector<float> top_key;
   uint32_t* d_matcap = nullptr;
   uint64_t matcap_capacity = 0;
   uint32_t matcap_w = 0, matcap_h = 0, cavity = 0;
@@ -208,6 +223,9 @@ struct Ctx {
   int fail(int code, const std::string& msg);
   int ensure_spill(uint64_t lanes, uint32_t entries);
   int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
+  int build_top_level(const uint32_t* slots, uint32_t n);
+  // top-level box widening for rays whose origins lie within origin_radius (max-norm) of the world origin
+  float top_delta(float origin_radius) const;
   int upload_object_table(const DevObject* tab, uint32_t n);
   // learned proposal for the frame's objects into fp.proposal (mlp.hip)
   int launch_mlp_frame(const FrameParams& fp, uint32_t n_obj);
@@ -247,6 +265,7 @@ struct Ctx {
     if (d_counters) hipFree(d_counters);
     if (spill) hipFree(spill);
     if (d_objs) hipFree(d_objs);
+    if (d_top) hipFree(d_top);
     if (d_matcap) hipFree(d_matcap);
     if (d_shadow) hipFree(d_shadow);
     d_shadow = nullptr;
@@ -261,6 +280,7 @@ struct Ctx {
     bframes_cap = bobjs_cap = 0;
     batch_tables.clear();
     d_queue = nullptr, d_counters = nullptr, spill = nullptr, d_objs = nullptr, d_matcap = nullptr;
+    d_top = nullptr;
   }
 };
 

@@ -270,6 +270,7 @@ struct Trav {
   float thi;    // culling bound: tmax * (1 + 2^-16)
   uint32_t cur; // next node / leaf reference, kNone when the stack must be popped
   uint32_t obj; // object index
+  uint32_t base;  // scenes with a top level: stack entries below this one are the top level's
   bool any;     // any-hit (shadow) mode
 };
 
@@ -319,11 +320,11 @@ __device__ inline bool test_triangle(Trav& s, Hit& best, Counters& c, const floa
 
 // Pops entries (culled against the current bound) until one is entered.
 // Returns true when the stack runs dry: this object's traversal is complete.
-template <int D>
+template <int D, bool TOP = false>
 __device__ inline bool pop_next(Trav& s, Stack<D>& st) {
   uint32_t ref;
   float dist;
-  while (st.pop(ref, dist)) {
+  while ((!TOP || st.sp > s.base) && st.pop(ref, dist)) {
     if (dist <= s.thi) {
       s.cur = ref;
       return false;
@@ -342,7 +343,7 @@ __device__ inline const float4* step_fetch(const Trav& s) {
 }
 
 // One traversal step of an 8-wide tree on the six words step_fetch pointed to.
-template <int D, bool COUNT>
+template <int D, bool COUNT, bool TOP = false>
 __device__ inline bool trav_step_fetched(Trav& s, Hit& best, Stack<D>& st, Counters& c, const float4 (&v)[6]) {
   if (COUNT) c.steps++;
   if (s.cur & kLeafBit) {
@@ -359,10 +360,10 @@ __device__ inline bool trav_step_fetched(Trav& s, Hit& best, Stack<D>& st, Count
   } else {
     s.cur = decode_node<8, D, COUNT>(v, s.r, s.thi, st, c, !s.any);
   }
-  return s.cur == kNone ? pop_next<D>(s, st) : false;
+  return s.cur == kNone ? pop_next<D, TOP>(s, st) : false;
 }
 
-template <int W, int D, bool COUNT>
+template <int W, int D, bool COUNT, bool TOP = false>
 __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
   const bool leaf = (s.cur & kLeafBit) != 0u;
   const uint32_t first = s.cur & kLeafIndexMask;
@@ -372,7 +373,7 @@ __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
     float4 v[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = gload4(p + k);
-    return trav_step_fetched<D, COUNT>(s, best, st, c, v);
+    return trav_step_fetched<D, COUNT, TOP>(s, best, st, c, v);
   } else {
     if (COUNT) c.steps++;
     if (leaf) {
@@ -390,7 +391,7 @@ __device__ inline bool trav_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
     } else {
       s.cur = visit_node<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
     }
-    return s.cur == kNone ? pop_next<D>(s, st) : false;
+    return s.cur == kNone ? pop_next<D, TOP>(s, st) : false;
   }
 }
 
@@ -589,7 +590,7 @@ __device__ inline uint32_t coop_visit(const float4* __restrict__ nodes, uint32_t
 
 // Cooperative step: as trav_step, lanes of a group splitting the node's
 // children or the leaf's triangles.  All lanes of the group end identical.
-template <int W, int D, bool COUNT>
+template <int W, int D, bool COUNT, bool TOP = false>
 __device__ inline bool coop_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) {
   const uint32_t j = group_lane();
   if (COUNT && j == 0) c.steps++;
@@ -633,18 +634,115 @@ __device__ inline bool coop_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
   } else {
     s.cur = coop_visit<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
   }
-  if (s.cur == kNone) {
-    uint32_t ref;
-    float dist;
-    while (st.pop(ref, dist)) {
-      if (dist <= s.thi) {
-        s.cur = ref;
-        return false;
+  return s.cur == kNone ? pop_next<D, TOP>(s, st) : false;
+}
+
+// ---- scenes of more than J3DG_MAX_OBJECTS objects: a top level ------------
+//
+// A binary BVH over the objects' world boxes (j3d's
+// qbvh_two_level_with_transformations, canvas.cpp:762): node 0 is the root;
+// a child ref with kLeafBit set is an object index.  Its entries share the
+// lane's stack: an object's traversal starts with base = the stack depth, and
+// pops stop there, so when it runs dry the top level's entries are next.  The
+// top level is walked here, inline, when a ray starts and whenever an object
+// is finished; lanes never fetch top-level nodes through the step machinery.
+// World boxes are tested widened by `delta` (a bound on how far a hit found
+// in an object's own frame can lie from the world ray, from rounding in the
+// transforms) on top of the usual t slack, so no object with a hit in range
+// is culled; results are independent of the order objects are visited in.
+constexpr uint32_t kTopGuard = 1u << 22;
+
+struct TopNode {
+  float box[2][6];  // the children's world boxes: lo xyz, hi xyz
+  uint32_t child[2];  // kNone for an empty slot (a one-object scene)
+  uint32_t pad[2];
+};
+
+// Entry distance of the world ray into child k's box widened by delta, or
+// INFINITY when it misses [tlo, thi].
+__device__ inline float top_entry(const TopNode* __restrict__ n, int k, const Ray& w, float delta, float tlo,
+                                  float thi) {
+  const float o[3] = {w.ox, w.oy, w.oz}, inv[3] = {w.ix, w.iy, w.iz};
+  float en = tlo, ex = thi;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float p = (gload(&n->box[k][a]) - delta - o[a]) * inv[a];
+    const float q = (gload(&n->box[k][3 + a]) + delta - o[a]) * inv[a];
+    en = fmaxf(en, fminf(p, q));
+    ex = fminf(ex, fmaxf(p, q) * kSlackUp);
+  }
+  return en <= ex ? en : INFINITY;
+}
+
+__device__ inline Ray world_ray(f4 o, f4 d, float tn, float tf) {
+  Ray r;
+  r.ox = o.x, r.oy = o.y, r.oz = o.z;
+  r.dx = d.x, r.dy = d.y, r.dz = d.z;
+  r.ix = safe_rcp(d.x), r.iy = safe_rcp(d.y), r.iz = safe_rcp(d.z);
+  r.t_near = tn;
+  r.t_far = tf;
+  return r;
+}
+
+// Walks the top level from s.cur (a top node, or kNone: pop the top level's
+// entries) to the next object whose box the ray enters within the current
+// bound, and starts that object's traversal (s.cur = its root).  Returns true
+// when no object is left: the ray is done.  The lane's stack holds top-level
+// entries only when this is called.
+template <int D>
+__device__ inline bool top_next(Trav& s, Stack<D>& st, const DevObject* __restrict__ objs,
+                                const TopNode* __restrict__ top, float delta, f4 wo, f4 wd) {
+  const Ray w = world_ray(wo, wd, s.r.t_near, s.r.t_far);
+  const float tlo = w.t_near * kSlackDown;
+  uint32_t cur = s.cur;
+  // each trip pops, enters an object or descends one level of a proper tree;
+  // the bound only guards against a malformed one (reported as an overflow)
+  for (uint32_t guard = 0;; ++guard) {
+    if (guard == kTopGuard) {
+      st.overflow = true;
+      s.cur = kNone;
+      return true;
+    }
+    if (cur == kNone) {
+      uint32_t ref;
+      float dist;
+      bool found = false;
+      while (st.pop(ref, dist)) {
+        if (dist <= s.thi) {
+          cur = ref;
+          found = true;
+          break;
+        }
+      }
+      if (!found) {
+        s.cur = kNone;
+        return true;
       }
     }
-    return true;
+    if (cur & kLeafBit) {
+      const uint32_t oi = cur & kLeafIndexMask;
+      const DevObject& ob = objs[oi];
+      s.r = make_object_ray(ob.inv_cs, wo, wd, w.t_near, w.t_far);
+      s.nodes = ob.nodes;
+      s.tris = ob.tris;
+      s.obj = oi;
+      s.base = st.sp;
+      s.cur = 0;  // the object's root, an internal node
+      return false;
+    }
+    const TopNode* n = top + cur;
+    const uint32_t c0 = gload(&n->child[0]), c1 = gload(&n->child[1]);
+    float e0 = c0 != kNone ? top_entry(n, 0, w, delta, tlo, s.thi) : INFINITY;
+    float e1 = c1 != kNone ? top_entry(n, 1, w, delta, tlo, s.thi) : INFINITY;
+    uint32_t r0 = c0, r1 = c1;
+    if (e1 < e0) {
+      const float te = e0;
+      e0 = e1, e1 = te;
+      r0 = c1, r1 = c0;
+    }
+    if (e1 != INFINITY) st.push(r1, e1);
+    cur = e0 != INFINITY ? r0 : kNone;
   }
-  return false;
 }
 
 }  // namespace j3d

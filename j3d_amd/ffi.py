@@ -51,7 +51,7 @@ class FrameDesc(C.Structure):
                 ("flags", C.c_uint32), ("light_dir", C.c_float * 3),
                 ("bg_top", C.c_uint32), ("bg_bottom", C.c_uint32),
                 ("n_objects", C.c_uint32), ("objects", C.c_uint32 * MAX_OBJECTS),
-                ("margin", C.c_float)]
+                ("margin", C.c_float), ("object_list", C.c_void_p)]
 
 
 class BuildOpts(C.Structure):

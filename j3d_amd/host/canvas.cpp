@@ -253,8 +253,13 @@ j3dg_frame_desc canvas::frame_desc(const scene* s, int x0, int y0, int x1, int y
     std::memcpy(d.cs_inv, s->coordinate_system_inv.data(), sizeof(d.cs_inv));
     std::memcpy(d.pivot, s->pivot, sizeof(d.pivot));
     d.diagonal = s->diagonal;
-    if (s->objects.size() > J3DG_MAX_OBJECTS) throw gpu_error(J3DG_ERR_UNSUPPORTED, "more than 16 scene objects");
-    for (const auto& o : s->objects) d.objects[d.n_objects++] = o.slot;
+    object_slots_.clear();
+    for (const auto& o : s->objects) object_slots_.push_back(o.slot);
+    d.n_objects = (uint32_t)object_slots_.size();
+    if (object_slots_.size() > J3DG_MAX_OBJECTS)
+      d.object_list = object_slots_.data();  // valid until the next frame description
+    else
+      std::copy(object_slots_.begin(), object_slots_.end(), d.objects);
   } else {
     d.flags |= J3DG_NO_SCENE;
   }

@@ -228,6 +228,8 @@ class canvas {
   canvas_settings _settings{};
   uint32_t bg_top_ = 0, bg_bottom_ = 0;
   bool directional_ = false, learned_ = false;
+  // the scene's mesh slots when it has more objects than j3dg_frame_desc::objects holds
+  mutable std::vector<uint32_t> object_slots_;
   float light_dir_[3] = {0, 0, 1}, margin_ = 0.f;
   const void* matcap_key_ = nullptr;
   uint32_t matcap_cavity_ = 0, matcap_w_ = 0;

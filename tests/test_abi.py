@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from j3d_amd import ffi
@@ -82,3 +83,18 @@ def test_null_arguments_are_rejected_without_a_device():
     o = ffi.TrainOpts()
     assert lib.j3dg_default_train_opts(C.byref(o)) == 0
     assert o.n_rays == 1 << 24 and o.batch % 128 == 0
+
+
+def test_frame_object_list():
+    """More objects than j3dg_frame_desc::objects holds go by object_list,
+    which the descriptor keeps alive; up to 16 fill objects[]."""
+    import j3d_amd as J
+    cam = J.camera_with_vertical_fov(60.0)
+    cs, ci = J.translation_camera(0.0, 0.0, 3.0)
+    small = J.make_frame(8, 8, cam, cs, ci, np.zeros(3, np.float32), 1.0, list(range(16))).desc()
+    assert small.n_objects == 16 and not small.object_list and list(small.objects) == list(range(16))
+    slots = [3 * i + 1 for i in range(40)]
+    big = J.make_frame(8, 8, cam, cs, ci, np.zeros(3, np.float32), 1.0, slots).desc()
+    assert big.n_objects == 40 and big.object_list
+    got = np.ctypeslib.as_array((C.c_uint32 * 40).from_address(big.object_list))
+    assert list(got) == slots
