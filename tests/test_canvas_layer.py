@@ -80,3 +80,53 @@ def test_canvas_frame_matches_oracle(tmp_path, case):
     og, oim = _oracle_frame(v, t, w, h, cam, cs, ci, flags, light, rect=rect)
     assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
     assert compare_rgba(im, oim)[0] <= 1
+
+
+@pytest.mark.gpu
+def test_canvas_many_objects_matches_oracle(tmp_path):
+    """A scene of 20 objects through the canvas layer: more than the frame
+    description's objects[] holds, so the canvas passes object_list and the
+    frame is traversed through the top-level BVH."""
+    w, h, n, spacing = 320, 160, 20, 0.9
+    v, t = J.icosphere(3, seed=6, amplitude=0.05)
+    cam = J.camera_with_vertical_fov(60.0)
+    cs, ci = J.orbit_camera(11.0, 1.5, 0.3, target=(n * spacing * 0.5, 0.0, 0.0))
+    flags = ffi.EDGES | ffi.SHADING | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    light = np.array([0.3, 0.5, 0.81], np.float32)
+    light /= np.linalg.norm(light)
+    prefix = str(tmp_path / "many")
+    cmd = [EXE, "--scene", "icosphere", "3", "6", "0.05", "--copies", str(n), repr(spacing),
+           "--size", str(w), str(h), "--focal", repr(float(cam.focal_length)),
+           "--cs"] + [repr(float(x)) for x in cs] + ["--flags", "edges,shading,shadow,directional", "--light"] + \
+        [repr(float(x)) for x in light] + ["--out", prefix]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = np.fromfile(prefix + ".gbuf", ffi.PIXEL_DTYPE).reshape(h, w)
+    im = np.fromfile(prefix + ".rgba", np.uint32).reshape(h, w)
+    # the oracle scene: object k translated by k * spacing (float32, as the harness computes it)
+    lo, hi = J.bbox(v)
+    meshes, blo, bhi = [], None, None
+    for k in range(n):
+        ocs = np.eye(4, dtype=np.float32)
+        ocs[3, 0] = np.float32(k) * np.float32(spacing)
+        ocs = ocs.reshape(16)
+        meshes.append(O.Mesh(v, t, db_id=(1 << 29) + k, cs=ocs))
+        a = (lo + np.array([ocs[12], 0, 0], np.float32)).astype(np.float32)
+        b = (hi + np.array([ocs[12], 0, 0], np.float32)).astype(np.float32)
+        blo = a if blo is None else np.minimum(blo, a)
+        bhi = b if bhi is None else np.maximum(bhi, b)
+    ext = bhi - blo  # prepare_scene (j3d/scene.cpp:50-89) over every object
+    diag = float(np.float32(max(ext[0], ext[1], ext[2])))
+    pivot = ((blo + bhi) * np.float32(0.5)).astype(np.float32)
+    fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, list(range(n)), flags=flags, light_dir=light)
+    mc, cav = O.matcap(0)
+    og = np.zeros((h, w), O.PIXEL_DTYPE)
+    og["object_id"] = 0xFFFFFFFF
+    og, _ = O.update_canvas(meshes, fr, gbuf=og)
+    oim = O.canvas_to_image(og, fr, mc, cav)
+    mm = compare_gbuffers(g, og)
+    assert all(c == 0 for c in mm.values()), mm
+    assert compare_rgba(im, oim)[0] <= 1
+    hit = g["object_id"] != 0xFFFFFFFF
+    assert len(np.unique(g["db_id"][hit])) > 16
