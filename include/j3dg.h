@@ -301,6 +301,22 @@ int j3dg_trace_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, cons
 int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
                       uint8_t* hit);
 
+/* All hits of each ray in one mesh (jtk::qbvh::find_all_triangles, called at j3d/vox.cpp:333): every
+   triangle whose test accepts t in [t_near, t_far], in (t, triangle id) order.  counts[i] = the ray's
+   total; the first max_hits hits go to [i * max_hits ...] of tri / t / u / v (each nullable; unused
+   slots: tri 0xFFFFFFFF, 0.f).  World rays as j3dg_trace_rays, taken to the mesh's frame by its cs.
+   n * max_hits <= 2^32, max_hits <= 65536. */
+int j3dg_all_hits(j3dg_ctx* ctx, uint32_t slot, const float* rays, uint64_t n, uint32_t max_hits, uint32_t* counts,
+                  uint32_t* tri, float* t, float* u, float* v);
+/* write_vox's voxel grid of a mesh (j3d/vox.cpp:278-376), in the mesh's own frame: dim from the mesh
+   box and max_dim (1 .. 1024) as vox.cpp:284-297; rays along x, y and z through every cell centre;
+   every hit's position and colour (texture over vertex colours over white, j3dg_set_mesh_colors) to
+   a voxel and a palette index (color_to_index, vox.cpp:154-171; index_to_color, vox.cpp:129-152, is
+   the palette).  data[x + (y + z * dim[1]) * dim[0]] (capacity bytes), 0 = empty.  Where several
+   hits land in one voxel, the sequentially last (axis, row, column, hit in (t, id) order) wins.
+   With data NULL only dim is returned.  Float-to-integer casts saturate into range (NaN -> 0). */
+int j3dg_voxelize(j3dg_ctx* ctx, uint32_t slot, uint32_t max_dim, uint32_t dim[3], uint8_t* data, uint64_t capacity);
+
 /* ---- measurement ---------------------------------------------------------
    A streaming device-to-device copy of `bytes` (read bytes, write bytes),
    repeated `repeats` times: the reference point that scales profiler byte

@@ -536,3 +536,27 @@ class Renderer:
         self._c(self.lib.j3dg_any_hit_rays(self.ctx, ffi.ptr(s), len(s), ffi.ptr(r), r.shape[0], ffi.ptr(hit)),
                 "j3dg_any_hit_rays")
         return hit
+
+    def all_hits(self, slot, rays, max_hits):
+        """Every hit of each ray in one mesh, (t, triangle id) order
+        (j3dg_all_hits): {"count": n, "tri"/"t"/"u"/"v": n x max_hits}."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        out = {"count": np.zeros(n, np.uint32), "tri": np.zeros((n, max_hits), np.uint32),
+               "t": np.zeros((n, max_hits), np.float32), "u": np.zeros((n, max_hits), np.float32),
+               "v": np.zeros((n, max_hits), np.float32)}
+        self._c(self.lib.j3dg_all_hits(self.ctx, int(slot), ffi.ptr(r), n, int(max_hits), ffi.ptr(out["count"]),
+                                       ffi.ptr(out["tri"]), ffi.ptr(out["t"]), ffi.ptr(out["u"]),
+                                       ffi.ptr(out["v"])), "j3dg_all_hits")
+        return out
+
+    def voxelize(self, slot, max_dim=100):
+        """write_vox's voxel grid of a mesh (j3dg_voxelize): (dims, uint8 array [z, y, x])."""
+        dim = np.zeros(3, np.uint32)
+        self._c(self.lib.j3dg_voxelize(self.ctx, int(slot), int(max_dim), ffi.ptr(dim), None, 0), "j3dg_voxelize")
+        n = int(dim[0]) * int(dim[1]) * int(dim[2])
+        data = np.zeros(n, np.uint8)
+        self._c(self.lib.j3dg_voxelize(self.ctx, int(slot), int(max_dim), ffi.ptr(dim), ffi.ptr(data), n),
+                "j3dg_voxelize")
+        dims = tuple(int(x) for x in dim)
+        return dims, data.reshape(dims[2], dims[1], dims[0])

@@ -43,15 +43,6 @@ constexpr int kStepsPerTrip = 3;
 // Lane phases of k_trace.
 enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u, kFinishPhase = 4u };
 
-// The byte a colour channel becomes, (uint8_t)(c * 255.f) of
-// j3d/canvas.cpp:829-831: truncation toward zero to int32 (NaN and values
-// outside the int32 range give 0), then the low 8 bits.
-__device__ inline uint32_t color_byte(float c) {
-  const float x = c * 255.f;
-  if (!(fabsf(x) < 2147483648.f)) return 0u;
-  return (uint32_t)(int32_t)x & 0xFFu;
-}
-
 // The first G-buffer word of a hit (j3d/canvas.cpp:801-834): mark 0, or mark
 // 2 with r, g, b in bytes 1-3 when the object has a surface colour under the
 // frame's flags (texture first, then vertex colours).  Sets `colored`.

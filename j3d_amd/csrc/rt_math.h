@@ -39,6 +39,15 @@ J3D_HD float dot4(f4 a, f4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w *
 
 J3D_HD uint32_t make_color(uint32_t r, uint32_t g, uint32_t b) { return 0xff000000u | (b << 16) | (g << 8) | r; }
 
+// The byte a colour channel becomes, (uint8_t)(c * 255.f) of
+// j3d/canvas.cpp:829-831 and j3d/vox.cpp:374: truncation toward zero to int32
+// (NaN and values outside the int32 range give 0), then the low 8 bits.
+J3D_HD uint32_t color_byte(float c) {
+  const float x = c * 255.f;
+  if (!(fabsf(x) < 2147483648.f)) return 0u;
+  return (uint32_t)(int32_t)x & 0xFFu;
+}
+
 // Moller-Trumbore ray/triangle test on (v0, e1 = v1-v0, e2 = v2-v0): two-sided,
 // det == 0 rejects, barycentric and [t_near, t_far] tests inclusive, NaN rejects.
 // This defines the test behind jtk's find_closest_triangle (j3d/canvas.cpp:786).

@@ -830,6 +830,163 @@ void orc_any_hits(const orc_object* objs, uint32_t nobj, const float* rays, uint
   });
 }
 
+/* ---------------------------------------------- all hits, voxels ---- */
+
+struct AnyHit4 {
+  float t, u, v;
+  uint32_t tri;
+};
+
+/* Every accepted hit of one object-space ray: the BVH (conservative, so the
+   same set as brute force) or, without one, every triangle. */
+static void all_hits_object(const orc_object& ob, const Ray& r, std::vector<AnyHit4>& out) {
+  out.clear();
+  auto consider = [&](uint32_t id, const float* v0, const float* e1, const float* e2) {
+    float t, u, v;
+    if (mt_test(r.o, r.d, v0, e1, e2, r.t_near, r.t_far, t, u, v)) out.push_back(AnyHit4{t, u, v, id});
+  };
+  if (!ob.bvh) {
+    for (uint64_t i = 0; i < ob.n_triangles; ++i) {
+      const uint32_t* tr = ob.triangles + 3 * i;
+      const float* a = ob.vertices + 3 * (size_t)tr[0];
+      const float* b = ob.vertices + 3 * (size_t)tr[1];
+      const float* c = ob.vertices + 3 * (size_t)tr[2];
+      float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      consider((uint32_t)i, a, e1, e2);
+    }
+  } else {
+    const orc_bvh& bvh = *ob.bvh;
+    std::vector<uint32_t> stack{0u};
+    while (!stack.empty()) {
+      const uint32_t ni = stack.back();
+      stack.pop_back();
+      const BNode& nd = bvh.nodes[ni];
+      if (box_entry(nd, r, r.t_far) == INFINITY) continue;
+      if (nd.b & LEAF_BIT) {
+        for (uint32_t k = 0; k < (nd.b & ~LEAF_BIT); ++k) {
+          const float* tr = &bvh.tri[9 * (size_t)(nd.a + k)];
+          consider(bvh.prim[nd.a + k], tr, tr + 3, tr + 6);
+        }
+      } else {
+        stack.push_back(nd.a);
+        stack.push_back(nd.b);
+      }
+    }
+  }
+  std::sort(out.begin(), out.end(),
+            [](const AnyHit4& x, const AnyHit4& y) { return x.t < y.t || (x.t == y.t && x.tri < y.tri); });
+}
+
+void orc_all_hits(const orc_object* ob, const float* rays, uint64_t n, uint32_t max_hits, int threads,
+                  uint32_t* counts, uint32_t* tri, float* t, float* u, float* v) {
+  std::vector<float> inv;
+  inv_cs_table(ob, 1, inv);
+  const uint64_t chunk = 256;
+  const uint64_t nchunks = (n + chunk - 1) / chunk;
+  parallel_rows(0, (uint32_t)nchunks, threads, [&](uint32_t ci) {
+    std::vector<AnyHit4> hits;
+    for (uint64_t i = ci * chunk; i < std::min(n, (ci + 1) * chunk); ++i) {
+      const float* q = rays + 8 * i;
+      const Ray r = to_object(*ob, inv.data(), f4{q[0], q[1], q[2], 1.f}, f4{q[3], q[4], q[5], 0.f}, q[6], q[7]);
+      all_hits_object(*ob, r, hits);
+      counts[i] = (uint32_t)hits.size();
+      for (uint32_t k = 0; k < max_hits; ++k) {
+        const size_t o = (size_t)i * max_hits + k;
+        const bool has = k < hits.size();
+        if (tri) tri[o] = has ? hits[k].tri : MISS;
+        if (t) t[o] = has ? hits[k].t : 0.f;
+        if (u) u[o] = has ? hits[k].u : 0.f;
+        if (v) v[o] = has ? hits[k].v : 0.f;
+      }
+    }
+  });
+}
+
+/* (uint32_t)(x) of a float that should lie in [0, n): truncation toward zero,
+   saturated into [0, n - 1]; NaN -> 0. */
+static uint32_t vox_index(float x, uint32_t n) {
+  if (!(x > 0.f)) return 0;
+  if (x >= (float)n) return n - 1;
+  return std::min((uint32_t)x, n - 1);
+}
+
+void orc_vox_dims(const float* mn, const float* mx, uint32_t max_dim, uint32_t dim[3]) {
+  int L = 0;
+  if ((mx[1] - mn[1]) > (mx[L] - mn[L])) L = 1;
+  if ((mx[2] - mn[2]) > (mx[L] - mn[L])) L = 2;
+  for (int j = 0; j < 3; ++j) {
+    const float q = (float)max_dim * (mx[j] - mn[j]) / (mx[L] - mn[L]);
+    uint32_t d = 0;
+    if (q > 0.f) d = q >= 4294967040.f ? 0xFFFFFF00u : (uint32_t)q;
+    dim[j] = d ? d : 1u;
+  }
+}
+
+uint8_t orc_vox_color_index(uint8_t r, uint8_t g, uint8_t b) {
+  r = r < 16 ? 0 : (uint8_t)(r - 16);
+  g = g < 16 ? 0 : (uint8_t)(g - 16);
+  b = b < 32 ? 0 : (uint8_t)(b - 32);
+  const uint8_t ret = (uint8_t)(((r >> 5) << 5) | ((g >> 5) << 2) | (b >> 6));
+  return ret ? ret : 1;
+}
+
+void orc_voxelize(const orc_object* ob, const float* mn, const float* mx, const uint32_t* dim, uint8_t* data) {
+  const uint64_t nvox = (uint64_t)dim[0] * dim[1] * dim[2];
+  std::memset(data, 0, nvox);
+  const bool use_texture = ob->uv && ob->texture && ob->tex_w > 0 && ob->tex_h > 0;
+  const bool use_vcol = !use_texture && ob->vertex_colors;
+  std::vector<AnyHit4> hits;
+  for (int dir = 0; dir < 3; ++dir) {
+    const int d1a = (dir + 1) % 3, d2a = (dir + 2) % 3;
+    Ray r;
+    for (int k = 0; k < 3; ++k) r.d[k] = 0.f;
+    r.d[dir] = dir == 2 ? 2.f : 1.f;
+    r.t_near = 0.f;
+    r.t_far = FLT_MAX;
+    set_inv(r);
+    for (uint32_t d1 = 0; d1 < dim[d1a]; ++d1)
+      for (uint32_t d2 = 0; d2 < dim[d2a]; ++d2) {
+        r.o[dir] = mn[dir];
+        r.o[d1a] = (((float)d1 + 0.5f) / (float)dim[d1a]) * (mx[d1a] - mn[d1a]) + mn[d1a];
+        r.o[d2a] = (((float)d2 + 0.5f) / (float)dim[d2a]) * (mx[d2a] - mn[d2a]) + mn[d2a];
+        all_hits_object(*ob, r, hits);
+        for (const AnyHit4& h : hits) {
+          const uint32_t* tr = ob->triangles + 3 * (size_t)h.tri;
+          const float* p0 = ob->vertices + 3 * (size_t)tr[0];
+          const float* p1 = ob->vertices + 3 * (size_t)tr[1];
+          const float* p2 = ob->vertices + 3 * (size_t)tr[2];
+          const float w = 1.f - h.u - h.v;
+          float pos[3], clr[3] = {1.f, 1.f, 1.f};
+          for (int k = 0; k < 3; ++k) pos[k] = p0[k] * w + h.u * p1[k] + h.v * p2[k];
+          if (use_texture) {
+            const float* uvc = ob->uv + 6 * (size_t)h.tri;
+            float c[2];
+            for (int k = 0; k < 2; ++k) {
+              c[k] = w * uvc[k] + h.u * uvc[2 + k] + h.v * uvc[4 + k];
+              c[k] = std::max(std::min(c[k], 1.f), 0.f);
+            }
+            const uint32_t x = vox_index(c[0] * (float)(ob->tex_w - 1), ob->tex_w);
+            const uint32_t y = vox_index(c[1] * (float)(ob->tex_h - 1), ob->tex_h);
+            const uint32_t col = ob->texture[(size_t)y * ob->tex_w + x];
+            clr[0] = (float)(col & 255) / 255.f;
+            clr[1] = (float)((col >> 8) & 255) / 255.f;
+            clr[2] = (float)((col >> 16) & 255) / 255.f;
+          } else if (use_vcol) {
+            const float* c0 = ob->vertex_colors + 3 * (size_t)tr[0];
+            const float* c1 = ob->vertex_colors + 3 * (size_t)tr[1];
+            const float* c2 = ob->vertex_colors + 3 * (size_t)tr[2];
+            for (int k = 0; k < 3; ++k) clr[k] = c0[k] * w + h.u * c1[k] + h.v * c2[k];
+          }
+          uint32_t X[3];
+          for (int k = 0; k < 3; ++k) X[k] = vox_index((pos[k] - mn[k]) / (mx[k] - mn[k]) * (float)dim[k], dim[k]);
+          data[X[0] + ((size_t)X[1] + (size_t)X[2] * dim[1]) * dim[0]] =
+              orc_vox_color_index(orc_color_byte(clr[0]), orc_color_byte(clr[1]), orc_color_byte(clr[2]));
+        }
+      }
+  }
+}
+
 uint8_t orc_color_byte(float c) {
   const float x = c * 255.f;
   if (!(std::fabs(x) < 2147483648.f)) return 0;

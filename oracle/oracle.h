@@ -148,6 +148,28 @@ void orc_closest_hits(const orc_object* objs, uint32_t nobj, const float* rays, 
 void orc_any_hits(const orc_object* objs, uint32_t nobj, const float* rays, uint64_t n, int threads,
                   uint8_t* hit);
 
+/* All hits of each ray in one object (jtk::qbvh::find_all_triangles, j3d/vox.cpp:333): every
+   triangle the test accepts with t in [t_near, t_far], in (t, triangle id) order; the first
+   max_hits are stored at [i * max_hits ...], counts[i] is the ray's total.  World rays, taken to
+   the object's frame like the closest-hit queries. */
+void orc_all_hits(const orc_object* ob, const float* rays, uint64_t n, uint32_t max_hits, int threads,
+                  uint32_t* counts, uint32_t* tri, float* t, float* u, float* v);
+
+/* write_vox's grid size (j3d/vox.cpp:284-297) for a mesh box: dim[k] = (uint32_t)(max_dim *
+   extent_k / largest extent) in float, at least 1 (a NaN ratio counts as 0). */
+void orc_vox_dims(const float bb_min[3], const float bb_max[3], uint32_t max_dim, uint32_t dim[3]);
+/* write_vox's voxel grid (j3d/vox.cpp:278-376) of one mesh in its own frame: rays along x, y and z
+   (direction (0,0,2) for z) through every cell centre of the two other axes, every hit's position
+   and colour (texture over vertex colours over white) mapped to a voxel and a palette index
+   (color_to_index, vox.cpp:154-171).  data[x + (y + z * dim[1]) * dim[0]], 0 = empty.  Where
+   several hits land in one voxel, the last in sequential order (axis, row, column, hit in (t, id)
+   order) wins.  Float-to-integer conversions saturate into range (NaN -> 0) where the reference's
+   casts would be undefined.  The object's cs is not applied. */
+void orc_voxelize(const orc_object* ob, const float bb_min[3], const float bb_max[3], const uint32_t dim[3],
+                  uint8_t* data);
+/* vox.cpp:154-171 */
+uint8_t orc_vox_color_index(uint8_t r, uint8_t g, uint8_t b);
+
 /* Single ray/triangle test, exposed for known-answer tests. Returns 1 on hit. */
 int orc_intersect_triangle(const float o[3], const float d[3], const float v0[3], const float v1[3],
                            const float v2[3], float t_near, float t_far, float* t, float* u, float* v);

@@ -85,6 +85,10 @@ def load():
                                        C.c_uint32, C.c_uint32, _P]),
         "orc_closest_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P, _P, _P, _P, _P, _P]),
         "orc_any_hits": (None, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, _P]),
+        "orc_all_hits": (None, [_P, _P, C.c_uint64, C.c_uint32, C.c_int, _P, _P, _P, _P, _P]),
+        "orc_vox_dims": (None, [_P, _P, C.c_uint32, _P]),
+        "orc_voxelize": (None, [_P, _P, _P, _P, _P]),
+        "orc_vox_color_index": (C.c_uint8, [C.c_uint8, C.c_uint8, C.c_uint8]),
         "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
         "orc_color_byte": (C.c_uint8, [C.c_float]),
         "orc_mlp_param_count": (C.c_int, []),
@@ -262,6 +266,43 @@ def closest_hits(meshes, rays, threads=0, brute=False, second=False):
     if second:
         out["t2"] = t2
     return out
+
+
+def all_hits(mesh, rays, max_hits, threads=0, brute=False):
+    """Every hit of each ray in one mesh, (t, triangle id) order (see oracle.h):
+    {"count": n, "tri"/"t"/"u"/"v": n x max_hits}, unused slots tri = MISS."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = r.shape[0]
+    out = {"count": np.zeros(n, np.uint32), "tri": np.zeros((n, max_hits), np.uint32),
+           "t": np.zeros((n, max_hits), np.float32), "u": np.zeros((n, max_hits), np.float32),
+           "v": np.zeros((n, max_hits), np.float32)}
+    arr, _ = _objs([mesh], brute)
+    load().orc_all_hits(arr, _p(r), n, int(max_hits), int(threads), _p(out["count"]), _p(out["tri"]),
+                        _p(out["t"]), _p(out["u"]), _p(out["v"]))
+    return out
+
+
+def vox_dims(bb_min, bb_max, max_dim):
+    lo = np.ascontiguousarray(bb_min, np.float32)
+    hi = np.ascontiguousarray(bb_max, np.float32)
+    d = np.zeros(3, np.uint32)
+    load().orc_vox_dims(_p(lo), _p(hi), int(max_dim), _p(d))
+    return tuple(int(x) for x in d)
+
+
+def voxelize(mesh, max_dim, brute=False):
+    """write_vox's voxel grid of one mesh (see oracle.h): (dims, uint8 array [z, y, x])."""
+    lo, hi = mesh.v.min(axis=0).astype(np.float32), mesh.v.max(axis=0).astype(np.float32)
+    dims = vox_dims(lo, hi, max_dim)
+    data = np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
+    arr, _ = _objs([mesh], brute)
+    d = np.array(dims, np.uint32)
+    load().orc_voxelize(arr, _p(lo), _p(hi), _p(d), _p(data))
+    return dims, data.reshape(dims[2], dims[1], dims[0])
+
+
+def vox_color_index(r, g, b):
+    return int(load().orc_vox_color_index(int(r), int(g), int(b)))
 
 
 def any_hits(meshes, rays, threads=0, brute=False):

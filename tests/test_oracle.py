@@ -545,3 +545,74 @@ def test_edge_pass_nan_normal_falls_back_to_unblended():
     # NaN makes |dot - 1| > 1e-4 false, angle = 0: the texel survives within truncation
     d = np.abs(np.array([im[0, 0]]).view(np.uint8).astype(int) - np.array([mc[V, U]]).view(np.uint8).astype(int))
     assert d.max() <= 1
+
+
+# ---- all hits (find_all_triangles) and the voxel grid (write_vox) ----------
+
+def _cube_mesh():
+    """The unit cube [0,1]^3 as 12 triangles."""
+    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
+    q = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    t = np.array([[a, b, c] for a, b, c, d in q] + [[a, c, d] for a, b, c, d in q], np.uint32)
+    return v, t
+
+
+def test_all_hits_bvh_equals_brute_force_and_is_sorted():
+    import j3d_amd as J
+    from scenes import random_rays
+    v, t = J.icosphere(3, seed=2, amplitude=0.2)
+    rays = random_rays(3000, 8, np.zeros(3), 1.2)
+    m = O.Mesh(v, t)
+    a = O.all_hits(m, rays, 16)
+    b = O.all_hits(m, rays, 16, brute=True)
+    for k in ("count", "tri", "t", "u", "v"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["count"].max() >= 2
+    for i in np.nonzero(a["count"] >= 2)[0][:200]:
+        n = min(int(a["count"][i]), 16)
+        pairs = list(zip(a["t"][i, :n], a["tri"][i, :n]))
+        assert pairs == sorted(pairs)
+    # the first hit is the closest hit; a ray hits something iff the any-hit query says so
+    ref = O.closest_hits([m], rays)
+    hit = a["count"] > 0
+    assert np.array_equal(hit, O.any_hits([m], rays).astype(bool))
+    assert np.array_equal(a["tri"][hit, 0], ref["tri"][hit])
+    assert np.array_equal(a["t"][hit, 0].view(np.uint32), ref["t"][hit].view(np.uint32))
+    # truncation keeps the first hits of the order
+    c = O.all_hits(m, rays, 1)
+    assert np.array_equal(c["count"], a["count"]) and np.array_equal(c["tri"][:, 0], a["tri"][:, 0])
+
+
+def test_vox_color_index_known_values():
+    # vox.cpp:154-171 by hand: r, g minus 16 and b minus 32 (floored at 0), then r>>5 <<5 | g>>5 <<2 | b>>6
+    assert O.vox_color_index(0, 0, 0) == 1  # index 0 is "empty": black maps to 1
+    assert O.vox_color_index(255, 255, 255) == (7 << 5) | (7 << 2) | 3
+    assert O.vox_color_index(255, 0, 0) == 7 << 5
+    assert O.vox_color_index(48, 48, 96) == (1 << 5) | (1 << 2) | 1
+    assert O.vox_color_index(47, 47, 95) == 1
+
+
+def test_vox_dims():
+    assert O.vox_dims([0, 0, 0], [2, 1, 0.5], 100) == (100, 50, 25)
+    assert O.vox_dims([0, 0, 0], [1, 3, 0], 10) == (3, 10, 1)  # a flat axis gets one cell
+    assert O.vox_dims([1, 1, 1], [1, 1, 1], 10) == (1, 1, 1)   # a point: 0/0 ratios count as 0
+
+
+def test_voxelize_cube_shell():
+    """Axis rays through the cell centres of the unit cube hit its faces only:
+    the voxels set are exactly the 8^3 grid's boundary shell, white (255)."""
+    v, t = _cube_mesh()
+    dims, data = O.voxelize(O.Mesh(v, t), 8)
+    assert dims == (8, 8, 8)
+    shell = np.ones((8, 8, 8), bool)
+    shell[1:-1, 1:-1, 1:-1] = False
+    assert np.array_equal(data > 0, shell)
+    assert set(np.unique(data[shell]).tolist()) == {255}
+    red = np.tile(np.array([[1.0, 0.0, 0.0]], np.float32), (len(v), 1))
+    _, data_r = O.voxelize(O.Mesh(v, t, vertex_colors=red), 8)
+    assert set(np.unique(data_r[shell]).tolist()) == {7 << 5}
+    # a texture wins over vertex colours: every texel blue (0xAABBGGRR)
+    tex = np.full((4, 4), 0xFFFF0000, np.uint32)
+    uv = np.tile(np.array([[0, 0, 1, 0, 0, 1]], np.float32), (len(t), 1))
+    _, data_t = O.voxelize(O.Mesh(v, t, vertex_colors=red, uv=uv, texture=tex), 8)
+    assert set(np.unique(data_t[shell]).tolist()) == {3}
