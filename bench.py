@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m", "set16x64"])
+    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m", "set16x64", "vox28m"])
     ap.add_argument("--grid", type=int, nargs=2, default=None, metavar=("NU", "NV"),
                     help="torus300m: torus grid (default 15000 10000 = 300M triangles)")
     ap.add_argument("--resolution", type=int, nargs=2, default=None, metavar=("W", "H"),
@@ -64,6 +64,7 @@ def parse():
                     help="max triangles per BVH leaf (0: the library default, 2 for 8-wide trees)")
     ap.add_argument("--builder", default="host", choices=["host", "device"],
                     help="BVH builder: binned SAH on the host (default) or the linear BVH built on the GPU")
+    ap.add_argument("--vox-dim", type=int, default=512, help="vox28m: max_dim of the voxel grid")
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--coop-tail", type=int, default=-1)
@@ -585,6 +586,64 @@ def mesh_set_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def voxel_bench(a):
+    """write_vox's voxel grid (j3d/vox.cpp:278-376) of the 28,000,000-triangle
+    torus at max_dim 512 (--vox-dim): rays along x, y and z through every cell
+    centre, every hit of each ray (find_all_triangles, vox.cpp:333), each hit to
+    its voxel and palette index.  value = voxel rays per second of the whole
+    j3dg_voxelize call (hit lists, voxel writes, grid to host), median of
+    --steps calls.  Parity: the grid at max_dim 96 against the oracle's."""
+    import j3d_amd as J
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    t0 = time.time()
+    v, t = J.torus(5000, 2800, seed=1)
+    log("torus 5000x2800: %d verts %d tris in %.1fs" % (len(v), len(t), time.time() - t0))
+    r = J.Renderer(0)
+    slot = r.upload_mesh(v, t, builder=BUILDERS[a.builder])
+    info = r.mesh_info(slot)
+    log("upload (%s build): build %.2fs copy %.2fs" % (a.builder, info["build_seconds"], info["upload_seconds"]))
+    md = a.vox_dim
+    for _ in range(max(1, a.warmup)):
+        dims, data = r.voxelize(slot, md)
+    times = []
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        dims, data = r.voxelize(slot, md)
+        times.append(time.perf_counter() - t1)
+    ms = 1e3 * float(np.median(times))
+    rays = dims[1] * dims[2] + dims[2] * dims[0] + dims[0] * dims[1]
+    log("voxelize max_dim %d: dims %s, %d rays, %d voxels set, %.2f ms" % (md, dims, rays, int((data > 0).sum()), ms))
+    parity, cpu = None, None
+    if not a.no_cpu_baseline and not a.profile_only:
+        import oracle as O
+        t2 = time.time()
+        m = O.Mesh(v, t)
+        build_s = time.time() - t2
+        pd = 96
+        gd, gdata = r.voxelize(slot, pd)
+        t3 = time.time()
+        od, odata = O.voxelize(m, pd)
+        cpu_s = time.time() - t3
+        parity = {"max_dim": pd, "dims": list(od), "voxels_differing": int((gdata != odata).sum()),
+                  "voxels_set": int((odata > 0).sum())}
+        crays = od[1] * od[2] + od[2] * od[0] + od[0] * od[1]
+        cpu = {"value": crays / cpu_s / 1e6, "unit": "M voxel rays/s", "cores": 1, "kind": "port",
+               "sample": "oracle write_vox restatement at max_dim %d (%d rays, %.1fs; oracle BVH build %.1fs "
+                         "excluded), one thread" % (pd, crays, cpu_s, build_s)}
+        log("parity at max_dim %d: %s; cpu %.3f M rays/s" % (pd, parity, cpu["value"]))
+    out = {"metric": "M voxel rays/s (write_vox grid of the 28M-triangle torus, max_dim %d)" % md,
+           "value": rays / (ms / 1e3) / 1e6, "unit": "M voxel rays/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "displaced torus R=1 r=0.35, 5000x2800 grid, 28,000,000 triangles; "
+                                  "write_vox voxel grid", "max_dim": md, "dims": list(dims), "rays": int(rays),
+                      "builder": a.builder},
+           "parity": parity, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -595,6 +654,8 @@ def main():
         return multi_camera(a)
     if a.workload == "set16x64":
         return mesh_set_bench(a)
+    if a.workload == "vox28m":
+        return voxel_bench(a)
     import j3d_amd as J
     from j3d_amd import ffi
     v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc = workload(a.workload)
