@@ -233,23 +233,6 @@ void vox_dims(const float* mn, const float* mx, uint32_t max_dim, uint32_t dim[3
   }
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() {
-    if (p) hipFree(p);
-  }
-  bool ensure(size_t b) {
-    if (b <= bytes) return true;
-    if (p) hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (hipMalloc(&p, std::max<size_t>(b, 16)) != hipSuccess) return false;
-    bytes = b;
-    return true;
-  }
-};
-
 void fill_mesh_args(const Mesh& m, AllHitsArgs& a) {
   std::memset(&a, 0, sizeof(a));
   a.nodes = m.d_nodes;
@@ -295,26 +278,26 @@ int j3dg_all_hits(j3dg_ctx* h, uint32_t slot, const float* rays, uint64_t n, uin
   if (n == 0) return J3DG_OK;
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   const Mesh& m = ctx->meshes[slot];
-  j3d::DevBuf d_rays, d_hits, d_counts;
-  if (!d_rays.ensure(n * 8 * sizeof(float)) || !d_hits.ensure(n * max_hits * sizeof(j3d::HitRec)) ||
-      !d_counts.ensure(n * sizeof(uint32_t)))
-    return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "all-hits buffers");
-  HIP_CALL(hipMemcpyAsync(d_rays.p, rays, n * 8 * sizeof(float), hipMemcpyHostToDevice, ctx->stream),
+  void* d_rays = ctx->scratch_buffer(0, n * 8 * sizeof(float));
+  void* d_hits = ctx->scratch_buffer(1, n * max_hits * sizeof(j3d::HitRec));
+  void* d_counts = ctx->scratch_buffer(2, n * sizeof(uint32_t));
+  if (!d_rays || !d_hits || !d_counts) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "all-hits buffers");
+  HIP_CALL(hipMemcpyAsync(d_rays, rays, n * 8 * sizeof(float), hipMemcpyHostToDevice, ctx->stream),
            J3DG_ERR_LAUNCH);
   j3d::AllHitsArgs a;
   j3d::fill_mesh_args(m, a);
-  a.rays = static_cast<const float*>(d_rays.p);
+  a.rays = static_cast<const float*>(d_rays);
   a.n = (uint32_t)n;
   a.cap = max_hits;
-  a.hits = static_cast<j3d::HitRec*>(d_hits.p);
-  a.counts = static_cast<uint32_t*>(d_counts.p);
+  a.hits = static_cast<j3d::HitRec*>(d_hits);
+  a.counts = static_cast<uint32_t*>(d_counts);
   int rc = j3d::launch_all_hits(ctx, m, a);
   if (rc) return rc;
-  HIP_CALL(hipMemcpyAsync(counts, d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+  HIP_CALL(hipMemcpyAsync(counts, d_counts, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
            J3DG_ERR_LAUNCH);
   std::vector<j3d::HitRec> hits((size_t)n * max_hits);
   if (max_hits)
-    HIP_CALL(hipMemcpyAsync(hits.data(), d_hits.p, hits.size() * sizeof(j3d::HitRec), hipMemcpyDeviceToHost,
+    HIP_CALL(hipMemcpyAsync(hits.data(), d_hits, hits.size() * sizeof(j3d::HitRec), hipMemcpyDeviceToHost,
                             ctx->stream),
              J3DG_ERR_LAUNCH);
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
@@ -342,16 +325,17 @@ int j3dg_voxelize(j3dg_ctx* h, uint32_t slot, uint32_t max_dim, uint32_t dim[3],
   if (!data) return J3DG_OK;
   if (capacity < nvox) return ctx->fail(J3DG_ERR_INVALID_ARG, "voxel buffer smaller than dim[0] * dim[1] * dim[2]");
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
-  j3d::DevBuf d_keys, d_hits, d_max, d_data;
-  if (!d_keys.ensure(nvox * sizeof(unsigned long long)) || !d_max.ensure(sizeof(uint32_t)) ||
-      !d_data.ensure(nvox))
-    return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel buffers");
-  HIP_CALL(hipMemsetAsync(d_keys.p, 0, nvox * sizeof(unsigned long long), ctx->stream), J3DG_ERR_LAUNCH);
+  // scratch 0: keys, then the uint8 grid after them; 1: hit lists; 2: the overflow count
+  void* d_keys = ctx->scratch_buffer(0, nvox * sizeof(unsigned long long) + nvox);
+  void* d_max = ctx->scratch_buffer(2, sizeof(uint32_t));
+  if (!d_keys || !d_max) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel buffers");
+  uint8_t* d_data = static_cast<uint8_t*>(d_keys) + nvox * sizeof(unsigned long long);
+  HIP_CALL(hipMemsetAsync(d_keys, 0, nvox * sizeof(unsigned long long), ctx->stream), J3DG_ERR_LAUNCH);
   j3d::AllHitsArgs a;
   j3d::fill_mesh_args(m, a);
   for (int k = 0; k < 3; ++k) a.dim[k] = dim[k], a.bmin[k] = m.info.bb_min[k], a.bmax[k] = m.info.bb_max[k];
-  a.keys = static_cast<unsigned long long*>(d_keys.p);
-  a.max_count = static_cast<uint32_t*>(d_max.p);
+  a.keys = static_cast<unsigned long long*>(d_keys);
+  a.max_count = static_cast<uint32_t*>(d_max);
   // texture over vertex colours over white (j3d/vox.cpp:301-304)
   if (m.d_uv && m.d_tex && m.tex_w && m.tex_h) {
     a.color_mode = 2, a.uv = m.d_uv, a.tex = m.d_tex, a.tex_w = m.tex_w, a.tex_h = m.tex_h;
@@ -362,17 +346,17 @@ int j3dg_voxelize(j3dg_ctx* h, uint32_t slot, uint32_t max_dim, uint32_t dim[3],
   for (uint32_t axis = 0; axis < 3; ++axis) {
     const uint32_t n = dim[(axis + 1) % 3] * dim[(axis + 2) % 3];
     for (;;) {
-      if (!d_hits.ensure((size_t)n * cap * sizeof(j3d::HitRec)))
-        return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel hit lists");
-      HIP_CALL(hipMemsetAsync(d_max.p, 0, sizeof(uint32_t), ctx->stream), J3DG_ERR_LAUNCH);
+      void* d_hits = ctx->scratch_buffer(1, (size_t)n * cap * sizeof(j3d::HitRec));
+      if (!d_hits) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel hit lists");
+      HIP_CALL(hipMemsetAsync(d_max, 0, sizeof(uint32_t), ctx->stream), J3DG_ERR_LAUNCH);
       a.axis = axis;
       a.n = n;
       a.cap = cap;
-      a.hits = static_cast<j3d::HitRec*>(d_hits.p);
+      a.hits = static_cast<j3d::HitRec*>(d_hits);
       int rc = j3d::launch_all_hits(ctx, m, a);
       if (rc) return rc;
       uint32_t mc = 0;
-      HIP_CALL(hipMemcpyAsync(&mc, d_max.p, sizeof(mc), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
+      HIP_CALL(hipMemcpyAsync(&mc, d_max, sizeof(mc), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
       HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
       if (mc <= cap) break;
       if (mc > j3d::kMaxVoxHits) return ctx->fail(J3DG_ERR_UNSUPPORTED, "more than 2^24 hits on one voxel ray");
@@ -381,9 +365,9 @@ int j3dg_voxelize(j3dg_ctx* h, uint32_t slot, uint32_t max_dim, uint32_t dim[3],
   }
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((nvox + 255) / 256, 65536);
   hipLaunchKernelGGL(j3d::k_vox_final, dim3(blocks), dim3(256), 0, ctx->stream,
-                     (const unsigned long long*)d_keys.p, nvox, static_cast<uint8_t*>(d_data.p));
+                     (const unsigned long long*)d_keys, nvox, d_data);
   HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
-  HIP_CALL(hipMemcpyAsync(data, d_data.p, nvox, hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
+  HIP_CALL(hipMemcpyAsync(data, d_data, nvox, hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;
 }

@@ -216,6 +216,9 @@ struct Ctx {
   uint32_t top_depth = 0;        // internal levels: stack entries the top level may hold
   float top_radius = 0.f;        // largest |coordinate| of the world boxes, translations and local boxes
   std::vector<float> top_key;
+  // device buffers of the all-hits / voxel calls, kept between calls (grown on demand)
+  void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_bytes[4] = {0, 0, 0, 0};
   uint32_t* d_matcap = nullptr;
   uint64_t matcap_capacity = 0;
   uint32_t matcap_w = 0, matcap_h = 0, cavity = 0;
@@ -224,6 +227,20 @@ struct Ctx {
   int ensure_spill(uint64_t lanes, uint32_t entries);
   int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
   int build_top_level(const uint32_t* slots, uint32_t n);
+  // scratch buffer i (0..3) of at least `bytes`, kept for later calls; nullptr when allocation fails
+  void* scratch_buffer(int i, size_t bytes) {
+    if (bytes <= scratch_bytes[i]) return scratch[i];
+    hipStreamSynchronize(stream);
+    if (scratch[i]) hipFree(scratch[i]);
+    scratch[i] = nullptr;
+    scratch_bytes[i] = 0;
+    if (hipMalloc(&scratch[i], std::max<size_t>(bytes, 256)) != hipSuccess) {
+      scratch[i] = nullptr;
+      return nullptr;
+    }
+    scratch_bytes[i] = std::max<size_t>(bytes, 256);
+    return scratch[i];
+  }
   // top-level box widening for rays whose origins lie within origin_radius (max-norm) of the world origin
   float top_delta(float origin_radius) const;
   int upload_object_table(const DevObject* tab, uint32_t n);
@@ -266,6 +283,8 @@ struct Ctx {
     if (spill) hipFree(spill);
     if (d_objs) hipFree(d_objs);
     if (d_top) hipFree(d_top);
+    for (int i = 0; i < 4; ++i)
+      if (scratch[i]) hipFree(scratch[i]), scratch[i] = nullptr, scratch_bytes[i] = 0;
     if (d_matcap) hipFree(d_matcap);
     if (d_shadow) hipFree(d_shadow);
     d_shadow = nullptr;

@@ -41,7 +41,7 @@ def _bases(small):
     return [J.torus(60, 30, seed=3), J.icosphere(3, seed=4, amplitude=0.05), J.torus(40, 70, r=0.2, seed=5)]
 
 
-def _scene(renderer, n, seed, spread, small=False, colors=False):
+def _scene(renderer, n, seed, spread, small=False, colors=False, width=8, builder=ffi.BUILD_HOST):
     rng = np.random.default_rng(seed)
     bases = _bases(small)
     slots, meshes = [], []
@@ -52,7 +52,7 @@ def _scene(renderer, n, seed, spread, small=False, colors=False):
         vc = None
         if colors and i % 2 == 0:
             vc = rng.uniform(0, 1, size=(len(v), 3)).astype(np.float32)
-        s = renderer.upload_mesh(v, t, db_id=db, cs=cs)
+        s = renderer.upload_mesh(v, t, db_id=db, cs=cs, width=width, builder=builder)
         if vc is not None:
             renderer.set_mesh_colors(s, vertex_colors=vc)
         slots.append(s)
@@ -189,4 +189,23 @@ def test_many_objects_learned_rejected(renderer):
         finally:
             renderer.destroy_target(tg)
     finally:
+        _remove(renderer, slots)
+
+
+@pytest.mark.parametrize("width, builder", [(4, ffi.BUILD_HOST), (8, ffi.BUILD_DEVICE), (4, ffi.BUILD_DEVICE)])
+def test_many_objects_widths_builders_counting(renderer, width, builder):
+    """The top level over 4-wide trees and device-built trees, rendered by the
+    counting instance of the kernel (per-ray statistics on)."""
+    spread = 2.0
+    slots, meshes = _scene(renderer, 30, 23, spread, width=width, builder=builder)
+    renderer.set_counting(True)
+    try:
+        fr = _frame(slots, spread, EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT, az=1.2, el=0.2)
+        g, im, og, oim, st = _render_both(renderer, meshes, fr)
+        _assert_parity(g, im, og, oim, "30 objects W%d builder %d" % (width, builder))
+        gst = renderer.stats()
+        assert gst["stack_overflow"] == 0
+        assert gst["primary_rays"] == fr.width * fr.height
+    finally:
+        renderer.set_counting(False)
         _remove(renderer, slots)
