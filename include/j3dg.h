@@ -211,7 +211,7 @@ int j3dg_upload_mesh(j3dg_ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n
                      uint64_t n_triangles, const float cs[16], const j3dg_build_opts* opts, uint32_t* slot);
 /* scene.cpp:43-52 remove_object */
 int j3dg_remove_mesh(j3dg_ctx* ctx, uint32_t slot);
-int j3dg_set_mesh_cs(j3dg_ctx* ctx, uint32_t slot, const float cs[16]);
+int j3dg_set_mesh_cs(j3dg_ctx* ctx, uint32_t slot, const float cs[16]);  /* cs finite, else invalid-arg */
 /* A mesh's surface colour, scene_object::p_vertex_colors / p_uv_coordinates /
    p_texture (scene.h:16-18, set by scene.cpp:17-19), used by update_canvas's
    textured and vertex-colour branches (canvas.cpp:803-834) under
@@ -295,7 +295,8 @@ int j3dg_mlp_adam_step(j3dg_ctx* ctx, float* params, float* m, float* v, const f
 
 /* ---- raw ray queries (world space; used by tests and by the learned proposal) ----
    rays: n x 8 floats (ox,oy,oz,dx,dy,dz,t_near,t_far) in host memory.
-   Closest hit over the listed mesh slots: lexicographic min of (t, object index, triangle id). */
+   Closest hit over the listed mesh slots: lexicographic min of (t, object index, triangle id).
+   A ray whose origin or direction is not finite hits nothing (in every query). */
 int j3dg_trace_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
                     uint32_t* obj, uint32_t* tri, float* t, float* u, float* v);
 int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
@@ -305,7 +306,7 @@ int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
    triangle whose test accepts t in [t_near, t_far], in (t, triangle id) order.  counts[i] = the ray's
    total; the first max_hits hits go to [i * max_hits ...] of tri / t / u / v (each nullable; unused
    slots: tri 0xFFFFFFFF, 0.f).  World rays as j3dg_trace_rays, taken to the mesh's frame by its cs.
-   n * max_hits <= 2^32, max_hits <= 65536. */
+   n * max_hits <= 2^28, max_hits <= 65536. */
 int j3dg_all_hits(j3dg_ctx* ctx, uint32_t slot, const float* rays, uint64_t n, uint32_t max_hits, uint32_t* counts,
                   uint32_t* tri, float* t, float* u, float* v);
 /* write_vox's voxel grid of a mesh (j3d/vox.cpp:278-376), in the mesh's own frame: dim from the mesh

@@ -118,9 +118,12 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
   const bool vox = a.rays == nullptr;
   uint32_t d1 = 0, d2 = 0;
   Trav s;
+  bool finite = true;
   if (!vox) {
     const float* q = a.rays + 8 * (size_t)i;
-    s.r = make_object_ray(a.inv_cs, f4{q[0], q[1], q[2], 1.f}, f4{q[3], q[4], q[5], 0.f}, q[6], q[7]);
+    const f4 o = {q[0], q[1], q[2], 1.f}, d = {q[3], q[4], q[5], 0.f};
+    finite = ray_finite(o, d);
+    s.r = make_object_ray(a.inv_cs, o, d, q[6], q[7]);
   } else {
     // write_vox's ray of row d1, column d2 (j3d/vox.cpp:317-330)
     const uint32_t ax = a.axis, ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3;
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
   s.tris = a.tris;
   s.thi = s.r.t_far * kSlackUp;
   s.tmax = s.r.t_far;
-  s.cur = 0;  // the root, an internal node
+  s.cur = finite ? 0u : kNone;  // the root, an internal node; a ray that is not finite hits nothing
   s.obj = 0;
   s.base = 0;
   s.any = false;
@@ -273,8 +276,8 @@ int j3dg_all_hits(j3dg_ctx* h, uint32_t slot, const float* rays, uint64_t n, uin
   if ((!rays || !counts) && n) return ctx->fail(J3DG_ERR_INVALID_ARG, "null rays or counts");
   if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
   if (n > (1ull << 31)) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many rays in one call");
-  if (max_hits > 65536 || n * (uint64_t)max_hits > (1ull << 32))
-    return ctx->fail(J3DG_ERR_INVALID_ARG, "n * max_hits exceeds 2^32 stored hits");
+  if (max_hits > 65536 || n * (uint64_t)max_hits > (1ull << 28))
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "n * max_hits exceeds 2^28 stored hits");
   if (n == 0) return J3DG_OK;
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   const Mesh& m = ctx->meshes[slot];

@@ -285,7 +285,8 @@ __global__ void k_collapse(CollapseArgs a) {
       if ((ch[i] & kPrimBit) || ref_count(a.nodes, ch[i]) <= a.leaf_size) continue;
       const BNode& c = a.nodes[ch[i]];
       const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
-      const float area = dx * dy + dy * dz + dz * dx;
+      float area = dx * dy + dy * dz + dz * dx;
+      if (!(area >= 0.f)) area = 0.f;  // a NaN box (NaN vertices) is still opened, so every level descends
       if (area > best_area) best_area = area, best = i;
     }
     if (best < 0) break;

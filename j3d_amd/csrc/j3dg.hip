@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <functional>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -368,8 +369,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       if (kIsFrame) pix_trips++;
       Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
       bool done;
-      if (lnobj == 0 || (kTop && phase != kFinishPhase && s.cur == kNone)) {
-        done = true;  // no object, or no object box on the ray's way
+      if (lnobj == 0 || (phase != kFinishPhase && s.cur == kNone)) {
+        done = true;  // no object, no object box on the ray's way, or a ray that is not finite
       } else if (kFrameLike && (phase == kFinishPhase || (W == 8 && !coop))) {
         // one fetch for every lane: a node, a leaf's triangles or a hit's completion record
         const float4* p = phase == kFinishPhase ? lobjs[best.obj].fin + 3 * (size_t)best.tri : step_fetch(s);
@@ -1585,6 +1586,9 @@ int j3dg_upload_mesh(j3dg_ctx* h, uint32_t db_id, const float* xyz, uint64_t n_v
   CHECK_CTX();
   if (!slot || (!xyz && n_vertices) || (!tris && n_triangles))
     return ctx->fail(J3DG_ERR_INVALID_ARG, "null mesh array");
+  if (cs)
+    for (int i = 0; i < 16; ++i)
+      if (!std::isfinite(cs[i])) return ctx->fail(J3DG_ERR_INVALID_ARG, "object pose is not finite");
   HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
   auto t0 = std::chrono::steady_clock::now();
   BuildOptions bo;
@@ -1747,6 +1751,8 @@ int j3dg_set_mesh_cs(j3dg_ctx* h, uint32_t slot, const float cs[16]) {
   CHECK_CTX();
   if (!cs) return ctx->fail(J3DG_ERR_INVALID_ARG, "null matrix");
   if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(cs[i])) return ctx->fail(J3DG_ERR_INVALID_ARG, "object pose is not finite");
   std::memcpy(ctx->meshes[slot].cs, cs, 16 * sizeof(float));
   invert_orthonormal(ctx->meshes[slot].cs, ctx->meshes[slot].inv_cs);
   return J3DG_OK;

@@ -274,6 +274,17 @@ struct Trav {
   bool any;     // any-hit (shadow) mode
 };
 
+// A ray whose origin or direction is not finite hits nothing (the triangle
+// test rejects it), and it must not be traversed either: its slab tests give
+// NaN, which fminf / fmaxf drop, so every child -- empty slots, whose ref is
+// kNone, included -- would count as entered.  Such rays start with cur = kNone
+// and finish as misses without a fetch.
+__device__ inline bool ray_finite(f4 o, f4 d) {
+  const float m = fmaxf(fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(d.x))),
+                        fmaxf(fabsf(d.y), fabsf(d.z)));
+  return m < INFINITY && o.x == o.x && o.y == o.y && o.z == o.z && d.x == d.x && d.y == d.y && d.z == d.z;
+}
+
 // Starts traversal of object `oi` for the world ray (o, d) over [tn, tf].
 __device__ inline void begin_object(Trav& s, const DevObject* __restrict__ objs, uint32_t oi, f4 o, f4 d, float tn,
                                     float tf, const Hit& best) {
@@ -284,7 +295,7 @@ __device__ inline void begin_object(Trav& s, const DevObject* __restrict__ objs,
   s.obj = oi;
   s.tmax = best.found ? fminf(best.t, tf) : tf;
   s.thi = s.tmax * kSlackUp;
-  s.cur = 0;  // the root is always an internal node
+  s.cur = ray_finite(o, d) ? 0u : kNone;  // the root is always an internal node
 }
 
 // One traversal step: visits s.cur (a wide node or a leaf), then advances
@@ -692,6 +703,10 @@ __device__ inline Ray world_ray(f4 o, f4 d, float tn, float tf) {
 template <int D>
 __device__ inline bool top_next(Trav& s, Stack<D>& st, const DevObject* __restrict__ objs,
                                 const TopNode* __restrict__ top, float delta, f4 wo, f4 wd) {
+  if (!ray_finite(wo, wd)) {
+    s.cur = kNone;
+    return true;
+  }
   const Ray w = world_ray(wo, wd, s.r.t_near, s.r.t_far);
   const float tlo = w.t_near * kSlackDown;
   uint32_t cur = s.cur;

@@ -149,8 +149,9 @@ def test_device_builder(renderer, width, leaf_size):
 def _degenerate_meshes():
     """Inputs that stress the device build: every Morton code equal (identical
     triangles: ties go to the lowest triangle id), a flat strip with one empty
-    axis and a 1e4:1 aspect, and meshes of one, two and three triangles (one
-    triangle is built on the host)."""
+    axis and a 1e4:1 aspect, a soup whose NaN vertices give NaN boxes, and
+    meshes of one, two and three triangles (one triangle is built on the
+    host)."""
     rng = np.random.default_rng(4)
     same_v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
     same_t = np.tile(np.array([[0, 1, 2]], np.uint32), (3000, 1))
@@ -160,16 +161,20 @@ def _degenerate_meshes():
     a = (np.arange(g)[None, :] + (g + 1) * np.arange(g)[:, None]).ravel().astype(np.uint32)
     strip_t = np.concatenate([np.stack([a, a + 1, a + g + 1], 1), np.stack([a + 1, a + g + 2, a + g + 1], 1)])
     small_v = rng.uniform(-1, 1, size=(9, 3)).astype(np.float32)
+    nan_v = rng.uniform(-1, 1, size=(3000, 3)).astype(np.float32)
+    nan_v[::7, rng.integers(0, 3)] = np.nan  # every 7th vertex has a NaN coordinate: its triangles never hit
+    nan_t = np.arange(3000, dtype=np.uint32).reshape(-1, 3)
     return [("identical x3000", same_v, same_t), ("flat strip", strip_v, strip_t.astype(np.uint32)),
+            ("soup with NaN vertices", nan_v, nan_t),
             ("1 tri", small_v, np.array([[0, 1, 2]], np.uint32)),
             ("2 tris", small_v, np.array([[0, 1, 2], [3, 4, 5]], np.uint32)),
             ("3 tris", small_v, np.arange(9, dtype=np.uint32).reshape(3, 3))]
 
 
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(6))
 def test_device_builder_degenerate(renderer, case):
     label, v, t = _degenerate_meshes()[case]
-    lo, hi = v.min(0), v.max(0)
+    lo, hi = np.nanmin(v, 0), np.nanmax(v, 0)
     center = (lo + hi) * 0.5
     radius = float(max(np.linalg.norm(hi - lo), 1e-3))
     # half the rays aimed at random points of random triangles, half at random points near the mesh
@@ -186,7 +191,7 @@ def test_device_builder_degenerate(renderer, case):
     aimed[:, 7] = np.finfo(np.float32).max
     rays = np.concatenate([aimed, random_rays(2000, 23 + case, center, radius * 0.6)])
     m = O.Mesh(v, t)
-    ref = O.closest_hits([m], rays)
+    ref = O.closest_hits([m], rays, brute=True)  # brute force: the oracle's own BVH needs no NaN handling
     for width in (4, 8):
         slot = renderer.upload_mesh(v, t, width=width, builder=ffi.BUILD_DEVICE)
         got = renderer.trace_rays([slot], rays)
@@ -194,9 +199,58 @@ def test_device_builder_degenerate(renderer, case):
             assert np.array_equal(got[k], ref[k]), (label, width, k)
         for k in ("t", "u", "v"):
             assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), (label, width, k)
-        assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays)), (label, width)
+        assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays, brute=True)), (label, width)
         renderer.remove_mesh(slot)
     print("%s: %d of %d rays hit" % (label, int((ref["obj"] != MISS).sum()), len(rays)))
+
+
+@pytest.mark.parametrize("builder", [ffi.BUILD_HOST, ffi.BUILD_DEVICE])
+def test_rays_that_are_not_finite_hit_nothing(renderer, builder):
+    """Rays with a NaN or infinite origin or direction component hit nothing
+    and are not traversed (their slab tests would be NaN, which min / max
+    drop): closest, any and all hits, with the finite rays of the same call
+    unchanged; also through the top level of a 17-object scene."""
+    v, t = J.torus(120, 60, seed=8)
+    rays = random_rays(3000, 29, np.zeros(3), 1.4)
+    bad = np.arange(0, 3000, 5)
+    comp = np.array([0, 1, 2, 3, 4, 5])[bad % 6]
+    vals = np.array([np.nan, np.inf, -np.inf], np.float32)[(bad // 6) % 3]
+    rays[bad, comp] = vals
+    slot = renderer.upload_mesh(v, t, builder=builder)
+    m = O.Mesh(v, t)
+    try:
+        got = renderer.trace_rays([slot], rays)
+        ref = O.closest_hits([m], rays, brute=True)
+        for k in ("obj", "tri"):
+            assert np.array_equal(got[k], ref[k]), k
+        assert (got["obj"][bad] == MISS).all()
+        assert np.array_equal(renderer.any_hit([slot], rays), O.any_hits([m], rays, brute=True))
+        ah = renderer.all_hits(slot, rays, 8)
+        assert (ah["count"][bad] == 0).all()
+        assert np.array_equal(ah["count"], O.all_hits(m, rays, 8, brute=True)["count"])
+        slots = [slot] + [renderer.upload_mesh(v, t, builder=builder) for _ in range(16)]
+        try:
+            got17 = renderer.trace_rays(slots, rays)
+            assert np.array_equal(got17["tri"], got["tri"]) and (got17["obj"][bad] == MISS).all()
+        finally:
+            for s2 in slots[1:]:
+                renderer.remove_mesh(s2)
+    finally:
+        renderer.remove_mesh(slot)
+
+
+def test_pose_must_be_finite(renderer):
+    v, t = J.torus(20, 10, seed=8)
+    cs = np.eye(4, dtype=np.float32).reshape(16)
+    cs[13] = np.nan
+    with pytest.raises(J.J3dgError):
+        renderer.upload_mesh(v, t, cs=cs)
+    slot = renderer.upload_mesh(v, t)
+    try:
+        with pytest.raises(J.J3dgError):
+            renderer.set_mesh_cs(slot, cs)
+    finally:
+        renderer.remove_mesh(slot)
 
 
 def test_mixed_builders_two_objects(renderer):
