@@ -369,8 +369,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       if (kIsFrame) pix_trips++;
       Counters& cc = (kFrameLike && phase == kShadowPhase) ? cs : c;
       bool done;
-      if (lnobj == 0 || (phase != kFinishPhase && s.cur == kNone)) {
-        done = true;  // no object, no object box on the ray's way, or a ray that is not finite
+      if (lnobj == 0 || (kTop && phase != kFinishPhase && s.cur == kNone)) {
+        done = true;  // no object, or (top level) no object box on the ray's way or a ray that is not finite
       } else if (kFrameLike && (phase == kFinishPhase || (W == 8 && !coop))) {
         // one fetch for every lane: a node, a leaf's triangles or a hit's completion record
         const float4* p = phase == kFinishPhase ? lobjs[best.obj].fin + 3 * (size_t)best.tri : step_fetch(s);

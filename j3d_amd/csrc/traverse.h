@@ -277,8 +277,9 @@ struct Trav {
 // A ray whose origin or direction is not finite hits nothing (the triangle
 // test rejects it), and it must not be traversed either: its slab tests give
 // NaN, which fminf / fmaxf drop, so every child -- empty slots, whose ref is
-// kNone, included -- would count as entered.  Such rays start with cur = kNone
-// and finish as misses without a fetch.
+// kNone, included -- would count as entered.  begin_object gives such a ray a
+// finite stand-in with an empty t range (t_near = +inf): the root's children
+// all test as not entered, so it visits the root only and finishes as a miss.
 __device__ inline bool ray_finite(f4 o, f4 d) {
   const float m = fmaxf(fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(d.x))),
                         fmaxf(fabsf(d.y), fabsf(d.z)));
@@ -289,13 +290,18 @@ __device__ inline bool ray_finite(f4 o, f4 d) {
 __device__ inline void begin_object(Trav& s, const DevObject* __restrict__ objs, uint32_t oi, f4 o, f4 d, float tn,
                                     float tf, const Hit& best) {
   const DevObject& ob = objs[oi];
+  if (!ray_finite(o, d)) {
+    o = f4{0.f, 0.f, 0.f, 1.f};
+    d = f4{1.f, 0.f, 0.f, 0.f};
+    tn = INFINITY;
+  }
   s.r = make_object_ray(ob.inv_cs, o, d, tn, tf);
   s.nodes = ob.nodes;
   s.tris = ob.tris;
   s.obj = oi;
   s.tmax = best.found ? fminf(best.t, tf) : tf;
   s.thi = s.tmax * kSlackUp;
-  s.cur = ray_finite(o, d) ? 0u : kNone;  // the root is always an internal node
+  s.cur = 0;  // the root is always an internal node
 }
 
 // One traversal step: visits s.cur (a wide node or a leaf), then advances
