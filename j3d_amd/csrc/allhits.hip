@@ -134,7 +134,9 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
     o[ax1] = (((float)d1 + 0.5f) / (float)a.dim[ax1]) * (a.bmax[ax1] - a.bmin[ax1]) + a.bmin[ax1];
     o[ax2] = (((float)d2 + 0.5f) / (float)a.dim[ax2]) * (a.bmax[ax2] - a.bmin[ax2]) + a.bmin[ax2];
     d[ax] = ax == 2 ? 2.f : 1.f;
-    s.r = world_ray(f4{o[0], o[1], o[2], 1.f}, f4{d[0], d[1], d[2], 0.f}, 0.f, FLT_MAX);
+    const f4 oo = {o[0], o[1], o[2], 1.f}, dd = {d[0], d[1], d[2], 0.f};
+    finite = ray_finite(oo, dd);  // a NaN or infinite mesh box gives rays that hit nothing
+    s.r = world_ray(oo, dd, 0.f, FLT_MAX);
   }
   s.nodes = a.nodes;
   s.tris = a.tris;

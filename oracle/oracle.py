@@ -292,7 +292,13 @@ def vox_dims(bb_min, bb_max, max_dim):
 
 def voxelize(mesh, max_dim, brute=False):
     """write_vox's voxel grid of one mesh (see oracle.h): (dims, uint8 array [z, y, x])."""
-    lo, hi = mesh.v.min(axis=0).astype(np.float32), mesh.v.max(axis=0).astype(np.float32)
+    # the mesh box as std::min / std::max folded from the first vertex
+    # (j3d/mesh.cpp:39-58): NaN coordinates are skipped, except in the first vertex
+    lo = np.fmin.reduce(mesh.v, axis=0).astype(np.float32)
+    hi = np.fmax.reduce(mesh.v, axis=0).astype(np.float32)
+    first_nan = np.isnan(mesh.v[0])
+    lo[first_nan] = np.nan
+    hi[first_nan] = np.nan
     dims = vox_dims(lo, hi, max_dim)
     data = np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
     arr, _ = _objs([mesh], brute)

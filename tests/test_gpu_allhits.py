@@ -110,3 +110,22 @@ def test_voxelize_long_hit_lists_and_device_build(renderer):
         assert (data > 0).sum() >= 50 * 20
     finally:
         renderer.remove_mesh(slot)
+
+
+def test_voxelize_mesh_box_not_finite(renderer):
+    """A NaN first vertex makes that axis of the mesh box NaN (the box is
+    std::min / std::max folded from the first vertex): one cell on that axis,
+    every voxel ray starts at a NaN coordinate and hits nothing."""
+    v, t = J.torus(30, 15, seed=2)
+    v = v.copy()
+    v[0, 1] = np.nan
+    slot = renderer.upload_mesh(v, t)
+    try:
+        dims, data = renderer.voxelize(slot, 32)
+        rdims, ref = O.voxelize(O.Mesh(v, t), 32)
+        assert dims == rdims and dims[1] == 1 and np.array_equal(data, ref) and not data.any()
+        rays = random_rays(500, 3, np.zeros(3), 1.4)
+        got = renderer.all_hits(slot, rays, 4)
+        assert np.array_equal(got["count"], O.all_hits(O.Mesh(v, t), rays, 4, brute=True)["count"])
+    finally:
+        renderer.remove_mesh(slot)
