@@ -239,6 +239,23 @@ def test_rays_that_are_not_finite_hit_nothing(renderer, builder):
         renderer.remove_mesh(slot)
 
 
+def test_frame_with_camera_that_is_not_finite(renderer):
+    """A NaN in the camera matrix makes every primary ray non-finite: the
+    frame traces no node beyond the root and comes out all background, as the
+    oracle's (whose triangle test rejects such rays)."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(100, 50, seed=3)
+    slot = renderer.upload_mesh(v, t, db_id=J.make_db_id(2, 2))
+    try:
+        cs = np.array(cs, np.float32)
+        cs[12] = np.nan
+        fr = J.make_frame(160, 120, cam, cs, ci, pivot, diag, [slot], flags=EDGES | ffi.SHADOW, light_dir=light)
+        g, im, og, oim, st = _render_both(renderer, None, [O.Mesh(v, t, db_id=J.make_db_id(2, 2))], fr)
+        _assert_parity(g, im, og, oim, "NaN camera")
+        assert (g["object_id"] == MISS).all()
+    finally:
+        renderer.remove_mesh(slot)
+
+
 def test_pose_must_be_finite(renderer):
     v, t = J.torus(20, 10, seed=8)
     cs = np.eye(4, dtype=np.float32).reshape(16)
