@@ -87,7 +87,7 @@ scene::~scene() {
 
 // j3d/scene.cpp:8-25
 void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<uint32_t>& triangles,
-                const float4x4& cs, uint32_t bvh_width) {
+                const float4x4& cs, uint32_t bvh_width, uint32_t builder) {
   if (vertices.size() % 3 || triangles.size() % 3) throw std::invalid_argument("add_object: arrays of xyz / ijk");
   scene_object obj;
   obj.db_id = id;
@@ -105,7 +105,7 @@ void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const
         obj.max_bb[k] = std::max(obj.max_bb[k], vertices[3 * i + k]);
       }
   }
-  j3dg_build_opts bo{bvh_width, 0, 0, J3DG_BUILD_HOST};
+  j3dg_build_opts bo{bvh_width, 0, 0, builder};
   s.dev->check(j3dg_upload_mesh(s.dev->ctx(), id, vertices.data(), nv, triangles.data(), triangles.size() / 3,
                                 cs.data(), &bo, &obj.slot),
                "j3dg_upload_mesh");

@@ -142,8 +142,10 @@ struct scene {
 
 // add_object (j3d/scene.cpp:8-39) for a mesh held by the caller: normals, bounding
 // box and BVH are computed by the device upload.  bvh_width: 4 or 8.
+// builder: J3DG_BUILD_HOST (binned SAH on the host) or J3DG_BUILD_DEVICE
+// (normals and a linear BVH built on the GPU from the raw arrays).
 void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<uint32_t>& triangles,
-                const float4x4& cs = get_identity(), uint32_t bvh_width = 8);
+                const float4x4& cs = get_identity(), uint32_t bvh_width = 8, uint32_t builder = J3DG_BUILD_HOST);
 // The surface colour sources of object `id`: scene_object::p_vertex_colors,
 // p_uv_coordinates and p_texture (j3d/scene.h:16-18, scene.cpp:17-19), used
 // under canvas_settings::vertexcolors / textured (canvas.cpp:803-834).

@@ -5,7 +5,7 @@
 //   j3d_render_frame --scene icosphere LEVEL SEED AMP | --scene torus NU NV SEED
 //                    --size W H [--focal MM] [--cs 16 floats] [--flags edges,shadow,...]
 //                    [--light X Y Z] [--matcap 0..3] [--bg TOP BOTTOM] [--rect X0 Y0 X1 Y1]
-//                    [--frames N] [--learned MARGIN] [--copies N SPACING] [--out PREFIX]
+//                    [--frames N] [--learned MARGIN] [--copies N SPACING] [--device-build] [--out PREFIX]
 // --copies: N objects of the mesh, object k translated by k * SPACING along x
 // (db_id 2^29 + k).
 // Prints one JSON line (sizes, times).  Without --cs the scene is framed by
@@ -54,6 +54,7 @@ int main(int argc, char** argv) {
   std::string kind, flags = "edges,shading", out;
   uint32_t level = 7, nu = 0, nv = 0, w = 0, h = 0, frames = 1, mc = 0, copies = 1;
   float spacing = 0.f;
+  uint32_t builder = J3DG_BUILD_HOST;
   uint64_t seed = 1;
   float amp = 0.05f, focal = 0.f, margin = -1.f;
   float cs[16];
@@ -124,6 +125,8 @@ int main(int argc, char** argv) {
       copies = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
       spacing = std::strtof(argv[++i], nullptr);
       if (!copies) usage("--copies needs at least one object");
+    } else if (a == "--device-build") {
+      builder = J3DG_BUILD_DEVICE;
     } else if (a == "--out") {
       need(1);
       out = argv[++i];
@@ -152,7 +155,7 @@ int main(int argc, char** argv) {
     for (uint32_t k = 0; k < copies; ++k) {
       float4x4 ocs = get_identity();
       ocs[12] = (float)k * spacing;
-      add_object((1u << 29) + k, sc, xyz, tri, ocs);
+      add_object((1u << 29) + k, sc, xyz, tri, ocs, 8, builder);
     }
     const auto t1 = std::chrono::steady_clock::now();
     prepare_scene(sc);

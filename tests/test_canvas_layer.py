@@ -83,10 +83,12 @@ def test_canvas_frame_matches_oracle(tmp_path, case):
 
 
 @pytest.mark.gpu
-def test_canvas_many_objects_matches_oracle(tmp_path):
+@pytest.mark.parametrize("device_build", [False, True])
+def test_canvas_many_objects_matches_oracle(tmp_path, device_build):
     """A scene of 20 objects through the canvas layer: more than the frame
     description's objects[] holds, so the canvas passes object_list and the
-    frame is traversed through the top-level BVH."""
+    frame is traversed through the top-level BVH; with --device-build the
+    objects' trees are built on the GPU (add_object's builder)."""
     w, h, n, spacing = 320, 160, 20, 0.9
     v, t = J.icosphere(3, seed=6, amplitude=0.05)
     cam = J.camera_with_vertical_fov(60.0)
@@ -98,7 +100,7 @@ def test_canvas_many_objects_matches_oracle(tmp_path):
     cmd = [EXE, "--scene", "icosphere", "3", "6", "0.05", "--copies", str(n), repr(spacing),
            "--size", str(w), str(h), "--focal", repr(float(cam.focal_length)),
            "--cs"] + [repr(float(x)) for x in cs] + ["--flags", "edges,shading,shadow,directional", "--light"] + \
-        [repr(float(x)) for x in light] + ["--out", prefix]
+        [repr(float(x)) for x in light] + ["--out", prefix] + (["--device-build"] if device_build else [])
     print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
