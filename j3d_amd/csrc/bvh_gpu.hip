@@ -8,9 +8,11 @@
 // internal node found independently from the longest common prefixes of its
 // neighbours); node boxes bottom-up, one launch per tree level (the level
 // lists come from a top-down pass; kernel boundaries order the levels, so no
-// thread waits for or fences against another); then, level by level from the root, each binary node is
-// opened into up to W children (the largest-area internal child first, as the
-// host collapser does), and written as the same quantized wide node, leaf
+// thread waits for or fences against another); the levels above a frontier of
+// some 16,000-32,000 subtrees rebuilt with a full-sweep SAH on the host over
+// those subtrees' boxes; then, level by level from the root, each binary node
+// is opened into up to W children (the largest-area internal child first, as
+// the host collapser does), and written as the same quantized wide node, leaf
 // encoding and 48-byte triangle records that build_bvh produces.  A subtree
 // of at most leaf_size triangles becomes a leaf: its triangles are contiguous
 // in Morton order.  The tree is of lower quality than the host's binned SAH
@@ -362,6 +364,130 @@ __global__ void k_normals_fin(const float* __restrict__ xyz, const uint32_t* __r
   for (int k = 0; k < 3; ++k) fin[3 * (size_t)i + k] = make_float4(v[k][0], v[k][1], v[k][2], nn[k]);
 }
 
+// ---- SAH over the top levels ------------------------------------------
+//
+// Morton order splits the top of the tree at spatial medians, where a bad
+// split costs the most.  The tree is cut at the frontier of maximal subtrees
+// of at most T = n / 2^14 triangles (some 16,000-32,000 clusters); the levels above it are
+// rebuilt on the host with a full-sweep SAH over the clusters' boxes, weighted
+// by their triangle counts, and written back into the binary nodes they
+// replace (exactly as many: a binary tree over m clusters has m - 1 internal
+// nodes), the root staying node 0.  A rebuilt node's triangles are not
+// contiguous in Morton order, so it is given a count above leaf_size: the
+// collapse always opens it.
+
+// Internal nodes of more than T triangles (the part rebuilt) and their
+// children of at most T (the frontier).
+__global__ void k_frontier(const BNode* __restrict__ nodes, uint32_t n, uint32_t T, uint32_t* __restrict__ frontier,
+                           uint32_t* __restrict__ top, uint32_t* __restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= n) return;
+  if (nodes[i].last - nodes[i].first + 1u <= T) return;
+  top[atomicAdd(&counters[1], 1u)] = i;
+  const uint32_t ch[2] = {nodes[i].left, nodes[i].right};
+  for (int k = 0; k < 2; ++k)
+    if (ref_count(nodes, ch[k]) <= T) frontier[atomicAdd(&counters[0], 1u)] = ch[k];
+}
+
+// Box and triangle count of each frontier entry: 8 words (lo, hi, count, ref).
+__global__ void k_frontier_boxes(const BNode* __restrict__ nodes, const float* __restrict__ box,
+                                 const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ frontier,
+                                 uint32_t m, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  float lo[3], hi[3];
+  child_box(nodes, box, sorted, frontier[i], lo, hi);
+  float* o = out + 8 * (size_t)i;
+  for (int k = 0; k < 3; ++k) o[k] = lo[k], o[3 + k] = hi[k];
+  o[6] = __uint_as_float(ref_count(nodes, frontier[i]));
+  o[7] = __uint_as_float(frontier[i]);
+}
+
+// Writes rebuilt nodes: nodes[index[j]] = val[j].
+__global__ void k_put_nodes(BNode* __restrict__ nodes, const uint32_t* __restrict__ index,
+                            const BNode* __restrict__ val, uint32_t k) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k) nodes[index[j]] = val[j];
+}
+
+constexpr bool kSahTop = true;
+
+struct Cluster {
+  float lo[3], hi[3];
+  float c[3];
+  uint32_t count, ref;
+};
+
+inline float half_area(const float* lo, const float* hi) {
+  const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  const float a = dx * dy + dy * dz + dz * dx;
+  return a >= 0.f ? a : 0.f;  // NaN boxes cost nothing (they hold no hits)
+}
+
+// Full-sweep SAH over cl[a, b) (b - a >= 2); appends the subtree's internal
+// nodes to `out` (its root first) and returns the root's position there.
+// Children refs: a cluster's own ref, or kPrimBit-free positions in `out`
+// tagged by `tag` (resolved to node indices by the caller).
+struct SahNode {
+  uint32_t left, right;  // cluster ref, or (kSahTag | position in out)
+  uint32_t count;
+  float lo[3], hi[3];
+};
+constexpr uint32_t kSahTag = 0x40000000u;
+
+inline uint32_t sah_build(std::vector<Cluster>& cl, uint32_t a, uint32_t b, std::vector<SahNode>& out,
+                          std::vector<float>& right_area) {
+  const uint32_t pos = (uint32_t)out.size();
+  out.emplace_back();
+  SahNode nd;
+  nd.count = 0;
+  for (int k = 0; k < 3; ++k) nd.lo[k] = INFINITY, nd.hi[k] = -INFINITY;
+  for (uint32_t i = a; i < b; ++i) {
+    nd.count += cl[i].count;
+    for (int k = 0; k < 3; ++k) nd.lo[k] = fminf(nd.lo[k], cl[i].lo[k]), nd.hi[k] = fmaxf(nd.hi[k], cl[i].hi[k]);
+  }
+  // best split over the three centroid orders
+  int best_axis = -1;
+  uint32_t best_mid = a + (b - a) / 2;
+  double best_cost = INFINITY;
+  for (int axis = 0; axis < 3; ++axis) {
+    std::sort(cl.begin() + a, cl.begin() + b, [axis](const Cluster& x, const Cluster& y) {
+      return x.c[axis] < y.c[axis] || (x.c[axis] == y.c[axis] && x.ref < y.ref);
+    });
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double cnt = 0.0;
+    for (uint32_t i = b; i-- > a + 1;) {  // right_area[i]: clusters [i, b)
+      for (int k = 0; k < 3; ++k) lo[k] = fminf(lo[k], cl[i].lo[k]), hi[k] = fmaxf(hi[k], cl[i].hi[k]);
+      cnt += cl[i].count;
+      right_area[i] = (float)((double)half_area(lo, hi) * cnt);
+    }
+    for (int k = 0; k < 3; ++k) lo[k] = INFINITY, hi[k] = -INFINITY;
+    cnt = 0.0;
+    for (uint32_t i = a; i + 1 < b; ++i) {  // split before i + 1
+      for (int k = 0; k < 3; ++k) lo[k] = fminf(lo[k], cl[i].lo[k]), hi[k] = fmaxf(hi[k], cl[i].hi[k]);
+      cnt += cl[i].count;
+      const double cost = (double)half_area(lo, hi) * cnt + (double)right_area[i + 1];
+      // ties go to the split nearest the middle (equal boxes: a balanced tree)
+      const uint32_t centre = a + (b - a) / 2;
+      const uint32_t dm = i + 1 > centre ? i + 1 - centre : centre - (i + 1);
+      const uint32_t db = best_mid > centre ? best_mid - centre : centre - best_mid;
+      if (cost < best_cost || (cost == best_cost && dm < db)) best_cost = cost, best_axis = axis, best_mid = i + 1;
+    }
+  }
+  if (best_axis != 2)  // the last sort was by z
+    std::sort(cl.begin() + a, cl.begin() + b, [best_axis](const Cluster& x, const Cluster& y) {
+      const int ax = best_axis < 0 ? 0 : best_axis;
+      return x.c[ax] < y.c[ax] || (x.c[ax] == y.c[ax] && x.ref < y.ref);
+    });
+  auto side = [&](uint32_t lo_i, uint32_t hi_i) {
+    return hi_i - lo_i == 1 ? cl[lo_i].ref : (kSahTag | sah_build(cl, lo_i, hi_i, out, right_area));
+  };
+  nd.left = side(a, best_mid);
+  nd.right = side(best_mid, b);
+  out[pos] = nd;
+  return pos;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -484,6 +610,77 @@ int device_build_mesh(Ctx* ctx, Mesh& m, const float* xyz, uint64_t n_vertices, 
   hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
                      reinterpret_cast<uint32_t*>(last.p), (uint64_t)(n + 3) / 4, 0u);
   HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+  // SAH over the levels above a frontier of 16k-32k clusters (meshes of 8192 triangles or more)
+  if (kSahTop && n >= 8192) {
+    const uint32_t T = std::max<uint32_t>(n >> 14, 2u);
+    uint32_t* frontier = reinterpret_cast<uint32_t*>(q0.p);  // n entries
+    uint32_t* top = reinterpret_cast<uint32_t*>(order.p);    // n - 1 entries
+    uint32_t* cnt2 = reinterpret_cast<uint32_t*>(counters.p);
+    HIP_TRY(hipMemsetAsync(cnt2, 0, 2 * sizeof(uint32_t), st), J3DG_ERR_LAUNCH);
+    hipLaunchKernelGGL(k_frontier, dim3((n - 1 + 255) / 256), dim3(256), 0, st, (const BNode*)bn, n, T, frontier, top,
+                       cnt2);
+    HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+    uint32_t mk[2] = {0u, 0u};
+    HIP_TRY(hipMemcpyAsync(mk, cnt2, sizeof(mk), hipMemcpyDeviceToHost, st), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipStreamSynchronize(st), J3DG_ERR_LAUNCH);
+    const uint32_t m = mk[0], k = mk[1];
+    if (m < 2 || k + 1 != m || m > n) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: malformed frontier");
+    Tmp fbox, put_idx, put_val;
+    if (!alloc(fbox, (size_t)m * 8 * sizeof(float)) || !alloc(put_idx, (size_t)k * sizeof(uint32_t)) ||
+        !alloc(put_val, (size_t)k * sizeof(BNode)))
+      return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "device BVH build: frontier buffers");
+    hipLaunchKernelGGL(k_frontier_boxes, dim3((m + 255) / 256), dim3(256), 0, st, (const BNode*)bn,
+                       (const float*)box.p, sorted, (const uint32_t*)frontier, m, (float*)fbox.p);
+    HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+    std::vector<float> fb(8 * (size_t)m);
+    std::vector<uint32_t> top_ids(k);
+    HIP_TRY(hipMemcpyAsync(fb.data(), fbox.p, fb.size() * sizeof(float), hipMemcpyDeviceToHost, st), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipMemcpyAsync(top_ids.data(), top, k * sizeof(uint32_t), hipMemcpyDeviceToHost, st), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipStreamSynchronize(st), J3DG_ERR_LAUNCH);
+    std::vector<Cluster> cl(m);
+    for (uint32_t i = 0; i < m; ++i) {
+      const float* f = &fb[8 * (size_t)i];
+      Cluster& c = cl[i];
+      for (int a = 0; a < 3; ++a) c.lo[a] = f[a], c.hi[a] = f[3 + a], c.c[a] = 0.5f * f[a] + 0.5f * f[3 + a];
+      std::memcpy(&c.count, f + 6, 4);
+      std::memcpy(&c.ref, f + 7, 4);
+    }
+    std::vector<SahNode> sn;
+    sn.reserve(k);
+    std::vector<float> right_area(m + 1);
+    sah_build(cl, 0, m, sn, right_area);
+    if (sn.size() != k) return ctx->fail(J3DG_ERR_INTERNAL, "device BVH build: top rebuild size");
+    // the rebuilt root stays binary node 0; the other rebuilt nodes take the replaced nodes' indices
+    std::vector<uint32_t> index_of(k);
+    uint32_t next = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+      if (j == 0) {
+        index_of[j] = 0;
+        continue;
+      }
+      while (top_ids[next] == 0) ++next;
+      index_of[j] = top_ids[next++];
+    }
+    std::vector<BNode> vals(k);
+    const uint32_t leaf_min = (uint32_t)leaf_size + 1u;
+    for (uint32_t j = 0; j < k; ++j) {
+      const SahNode& s2 = sn[j];
+      BNode& b = vals[j];
+      auto resolve = [&](uint32_t r) { return (r & kSahTag) && !(r & kPrimBit) ? index_of[r & ~kSahTag] : r; };
+      b.left = resolve(s2.left);
+      b.right = resolve(s2.right);
+      b.first = 0;
+      b.last = std::max(s2.count, leaf_min) - 1u;  // never a leaf: its triangles are not contiguous
+      for (int a = 0; a < 3; ++a) b.lo[a] = s2.lo[a], b.hi[a] = s2.hi[a];
+    }
+    HIP_TRY(hipMemcpyAsync(put_idx.p, index_of.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+            J3DG_ERR_LAUNCH);
+    HIP_TRY(hipMemcpyAsync(put_val.p, vals.data(), k * sizeof(BNode), hipMemcpyHostToDevice, st), J3DG_ERR_LAUNCH);
+    hipLaunchKernelGGL(k_put_nodes, dim3((k + 255) / 256), dim3(256), 0, st, bn, (const uint32_t*)put_idx.p,
+                       (const BNode*)put_val.p, k);
+    HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipStreamSynchronize(st), J3DG_ERR_LAUNCH);
+  }
   // collapse, level by level from the root (wide node 0 <- binary node 0)
   uint32_t host_c[4] = {1u, 1u, 0u, 0u};
   Item root{0u, 0u, 0u};
