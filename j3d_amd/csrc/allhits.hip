@@ -43,9 +43,9 @@ struct AllHitsArgs {
   const float4* nodes;
   const float4* tris;
   float inv_cs[16];         // world -> object for query rays
-  const float* rays;        // n x 8 world rays; nullptr: the voxel rays of `axis`
+  const float* rays;        // n x 8 world rays; nullptr: the voxel rays of all three axes
   uint32_t n;
-  uint32_t axis;            // voxel rays: along x, y or z
+  uint32_t axis_first[3];   // voxel rays: first ray of the x, y and z axis (x rays start at 0)
   uint32_t dim[3];
   float bmin[3], bmax[3];
   uint32_t cap;             // stored hits per ray
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const bool vox = a.rays == nullptr;
-  uint32_t d1 = 0, d2 = 0;
+  uint32_t d1 = 0, d2 = 0, ax = 0;
   Trav s;
   bool finite = true;
   if (!vox) {
@@ -125,10 +125,11 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
     finite = ray_finite(o, d);
     s.r = make_object_ray(a.inv_cs, o, d, q[6], q[7]);
   } else {
-    // write_vox's ray of row d1, column d2 (j3d/vox.cpp:317-330)
-    const uint32_t ax = a.axis, ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3;
-    d1 = i / a.dim[ax2];
-    d2 = i % a.dim[ax2];
+    // write_vox's ray of row d1, column d2 along axis ax (j3d/vox.cpp:317-330)
+    ax = i >= a.axis_first[2] ? 2u : (i >= a.axis_first[1] ? 1u : 0u);
+    const uint32_t ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3, r = i - a.axis_first[ax];
+    d1 = r / a.dim[ax2];
+    d2 = r % a.dim[ax2];
     float o[3], d[3] = {0.f, 0.f, 0.f};
     o[ax] = a.bmin[ax];
     o[ax1] = (((float)d1 + 0.5f) / (float)a.dim[ax1]) * (a.bmax[ax1] - a.bmin[ax1]) + a.bmin[ax1];
@@ -171,7 +172,7 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
   }
   if (a.counts) a.counts[i] = count;
   if (!vox) return;
-  if (count > a.cap) {  // the host grows the lists and runs this axis again
+  if (count > a.cap) {  // the host grows the lists and runs the launch again
     atomicMax(a.max_count, count);
     return;
   }
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(kAllBlock) k_all_hits(AllHitsArgs a) {
     for (int j = 0; j < 3; ++j)
       X[j] = vox_index((pos[j] - a.bmin[j]) / (a.bmax[j] - a.bmin[j]) * (float)a.dim[j], a.dim[j]);
     const uint32_t ci = vox_color_index(color_byte(clr[0]), color_byte(clr[1]), color_byte(clr[2]));
-    unsigned long long key = a.axis;
+    unsigned long long key = ax;
     key = (key << 12) | d1;
     key = (key << 12) | d2;
     key = (key << 24) | k;
@@ -347,26 +348,27 @@ int j3dg_voxelize(j3dg_ctx* h, uint32_t slot, uint32_t max_dim, uint32_t dim[3],
   } else if (m.d_vcol) {
     a.color_mode = 1, a.vcol = m.d_vcol;
   }
+  // the three axes' rays in one launch: x rays, then y, then z
+  a.axis_first[0] = 0;
+  a.axis_first[1] = dim[1] * dim[2];
+  a.axis_first[2] = a.axis_first[1] + dim[2] * dim[0];
+  const uint32_t n = a.axis_first[2] + dim[0] * dim[1];
   uint32_t cap = 32;
-  for (uint32_t axis = 0; axis < 3; ++axis) {
-    const uint32_t n = dim[(axis + 1) % 3] * dim[(axis + 2) % 3];
-    for (;;) {
-      void* d_hits = ctx->scratch_buffer(1, (size_t)n * cap * sizeof(j3d::HitRec));
-      if (!d_hits) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel hit lists");
-      HIP_CALL(hipMemsetAsync(d_max, 0, sizeof(uint32_t), ctx->stream), J3DG_ERR_LAUNCH);
-      a.axis = axis;
-      a.n = n;
-      a.cap = cap;
-      a.hits = static_cast<j3d::HitRec*>(d_hits);
-      int rc = j3d::launch_all_hits(ctx, m, a);
-      if (rc) return rc;
-      uint32_t mc = 0;
-      HIP_CALL(hipMemcpyAsync(&mc, d_max, sizeof(mc), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
-      HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
-      if (mc <= cap) break;
-      if (mc > j3d::kMaxVoxHits) return ctx->fail(J3DG_ERR_UNSUPPORTED, "more than 2^24 hits on one voxel ray");
-      while (cap < mc) cap *= 2;  // the rays that fitted wrote their voxels already; writing them again is harmless
-    }
+  for (;;) {
+    void* d_hits = ctx->scratch_buffer(1, (size_t)n * cap * sizeof(j3d::HitRec));
+    if (!d_hits) return ctx->fail(J3DG_ERR_OUT_OF_MEMORY, "voxel hit lists");
+    HIP_CALL(hipMemsetAsync(d_max, 0, sizeof(uint32_t), ctx->stream), J3DG_ERR_LAUNCH);
+    a.n = n;
+    a.cap = cap;
+    a.hits = static_cast<j3d::HitRec*>(d_hits);
+    int rc = j3d::launch_all_hits(ctx, m, a);
+    if (rc) return rc;
+    uint32_t mc = 0;
+    HIP_CALL(hipMemcpyAsync(&mc, d_max, sizeof(mc), hipMemcpyDeviceToHost, ctx->stream), J3DG_ERR_LAUNCH);
+    HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+    if (mc <= cap) break;
+    if (mc > j3d::kMaxVoxHits) return ctx->fail(J3DG_ERR_UNSUPPORTED, "more than 2^24 hits on one voxel ray");
+    while (cap < mc) cap *= 2;  // the rays that fitted wrote their voxels already; writing them again is harmless
   }
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((nvox + 255) / 256, 65536);
   hipLaunchKernelGGL(j3d::k_vox_final, dim3(blocks), dim3(256), 0, ctx->stream,
