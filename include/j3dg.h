@@ -161,7 +161,9 @@ typedef struct j3dg_tuning {
   uint32_t split_passes;   /* 0 = one launch per frame (a lane traces its pixel's shadow ray right after
                               the primary hit); 1 = primary pass, hit completion, shadow pass */
   uint32_t cost_order;     /* 1 = hand out a target's tiles heaviest first, by the work each tile took in
-                              the target's previous frame with the same rectangle (results unchanged) */
+                              the target's previous frame with the same rectangle (results unchanged);
+                              rectangles of fewer than 8192 8x8 tiles (about 720x720 pixels) skip it:
+                              there the three sorting launches cost more than the shorter tail saves */
 } j3dg_tuning;
 
 /* Statistics of the last j3dg_render / j3dg_render_batch. */

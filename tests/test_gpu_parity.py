@@ -345,13 +345,14 @@ def test_cost_ordered_tiles_repeat_frames(renderer):
     meshes = [O.Mesh(v, t, db_id=J.make_db_id(1, 0))]
     mc, cav = J.make_matcap(0)
     renderer.set_matcap(mc, cav)
-    tg = renderer.create_target(512, 288)
+    # 1024x576 = 9216 tiles and a rectangle of 8487: both at least the 8192 tiles given a cost order
+    tg = renderer.create_target(1024, 576)
     flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
     try:
-        for k, rect in enumerate([None, None, None, (30, 20, 400, 250), (30, 20, 400, 250), None]):
-            fr = J.make_frame(512, 288, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light, rect=rect)
+        for k, rect in enumerate([None, None, None, (20, 10, 1000, 560), (20, 10, 1000, 560), None]):
+            fr = J.make_frame(1024, 576, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light, rect=rect)
             g, im = renderer.render(fr, tg)
-            og = np.zeros((288, 512), O.PIXEL_DTYPE)
+            og = np.zeros((576, 1024), O.PIXEL_DTYPE)
             og["object_id"] = 0xFFFFFFFF
             # the oracle G-buffer carries over like the target's: render into the previous result
             if k:
