@@ -1119,18 +1119,39 @@ int Ctx::upload_object_table(const DevObject* tab, uint32_t n) {
   if (n && n * sizeof(DevObject) == obj_cache.size() &&
       std::memcmp(obj_cache.data(), tab, obj_cache.size()) == 0)
     return J3DG_OK;
-  if (n > obj_capacity) {
-    if (d_objs) hipFree(d_objs);
-    d_objs = nullptr;
-    obj_capacity = 0;
-    Ctx* ctx = this;
-    HIP_TRY(hipMalloc(&d_objs, std::max<uint32_t>(n, 16) * sizeof(DevObject)), J3DG_ERR_OUT_OF_MEMORY);
-    obj_capacity = std::max<uint32_t>(n, 16);
-  }
-  if (n) {
-    Ctx* ctx = this;
+  Ctx* ctx = this;
+  if (n && n <= J3DG_MAX_OBJECTS) {
+    if (!d_obj_ring) {
+      HIP_TRY(hipMalloc(&d_obj_ring, (size_t)kObjRing * J3DG_MAX_OBJECTS * sizeof(DevObject)), J3DG_ERR_OUT_OF_MEMORY);
+      HIP_TRY(hipHostMalloc(&h_obj_ring, (size_t)kObjRing * J3DG_MAX_OBJECTS * sizeof(DevObject)), J3DG_ERR_OUT_OF_MEMORY);
+      for (int i = 0; i < kObjRing; ++i)
+        HIP_TRY(hipEventCreateWithFlags(&obj_ring_ev[i], hipEventDisableTiming), J3DG_ERR_LAUNCH);
+    }
+    // retire the slot in use: every kernel that reads it has been enqueued already
+    if (obj_ring_cur >= 0) {
+      HIP_TRY(hipEventRecord(obj_ring_ev[obj_ring_cur], stream), J3DG_ERR_LAUNCH);
+      obj_ring_used[obj_ring_cur] = true;
+    }
+    const int j = (obj_ring_cur + 1) % kObjRing;
+    if (obj_ring_used[j]) HIP_TRY(hipEventSynchronize(obj_ring_ev[j]), J3DG_ERR_LAUNCH);
+    DevObject* h = h_obj_ring + (size_t)j * J3DG_MAX_OBJECTS;
+    DevObject* d = d_obj_ring + (size_t)j * J3DG_MAX_OBJECTS;
+    std::memcpy(h, tab, n * sizeof(DevObject));
+    HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(DevObject), hipMemcpyHostToDevice, stream), J3DG_ERR_LAUNCH);
+    obj_ring_cur = j;
+    d_objs = d;
+  } else if (n) {
+    if (n > obj_capacity) {
+      HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
+      if (d_obj_big) hipFree(d_obj_big);
+      d_obj_big = nullptr;
+      obj_capacity = 0;
+      HIP_TRY(hipMalloc(&d_obj_big, (size_t)n * sizeof(DevObject)), J3DG_ERR_OUT_OF_MEMORY);
+      obj_capacity = n;
+    }
     HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
-    HIP_TRY(hipMemcpy(d_objs, tab, n * sizeof(DevObject), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+    HIP_TRY(hipMemcpy(d_obj_big, tab, n * sizeof(DevObject), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
+    d_objs = d_obj_big;
   }
   obj_cache.assign(reinterpret_cast<const uint8_t*>(tab), reinterpret_cast<const uint8_t*>(tab) + n * sizeof(DevObject));
   return J3DG_OK;

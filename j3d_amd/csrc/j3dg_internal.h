@@ -206,8 +206,19 @@ struct Ctx {
   uint64_t* d_counters = nullptr;
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
-  DevObject* d_objs = nullptr;
-  uint32_t obj_capacity = 0;
+  DevObject* d_objs = nullptr;   // the current object table (not owned: a ring slot or d_obj_big)
+  // tables of up to J3DG_MAX_OBJECTS objects go through a ring of device
+  // slots filled asynchronously from pinned staging, so that a frame whose
+  // objects differ from the last one's does not wait for the GPU; a slot is
+  // rewritten only after the event recorded when it was retired has passed
+  static constexpr int kObjRing = 8;
+  DevObject* d_obj_ring = nullptr;  // kObjRing x J3DG_MAX_OBJECTS
+  DevObject* h_obj_ring = nullptr;  // pinned, same shape
+  hipEvent_t obj_ring_ev[kObjRing] = {};
+  bool obj_ring_used[kObjRing] = {};
+  int obj_ring_cur = -1;
+  DevObject* d_obj_big = nullptr;   // larger tables (top-level scenes), written synchronously
+  uint32_t obj_capacity = 0;        // entries of d_obj_big
   std::vector<uint8_t> obj_cache;
   // top-level BVH of the last object table of more than J3DG_MAX_OBJECTS
   // objects (TopNode[]), keyed by the objects' boxes and poses
@@ -281,7 +292,14 @@ struct Ctx {
     if (d_queue) hipFree(d_queue);
     if (d_counters) hipFree(d_counters);
     if (spill) hipFree(spill);
-    if (d_objs) hipFree(d_objs);
+    if (d_obj_big) hipFree(d_obj_big);
+    if (d_obj_ring) hipFree(d_obj_ring);
+    if (h_obj_ring) hipHostFree(h_obj_ring);
+    for (int i = 0; i < kObjRing; ++i)
+      if (obj_ring_ev[i]) hipEventDestroy(obj_ring_ev[i]), obj_ring_ev[i] = nullptr;
+    d_obj_big = nullptr, d_obj_ring = nullptr, h_obj_ring = nullptr, obj_capacity = 0, obj_ring_cur = -1;
+    for (int i = 0; i < kObjRing; ++i) obj_ring_used[i] = false;
+    obj_cache.clear();
     if (d_top) hipFree(d_top);
     for (int i = 0; i < 4; ++i)
       if (scratch[i]) hipFree(scratch[i]), scratch[i] = nullptr, scratch_bytes[i] = 0;
