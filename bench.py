@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m", "set16x64", "vox28m"])
+    ap.add_argument("--workload", default="torus28m", choices=["torus28m", "icosphere327k", "torus300m", "set16x64", "vox28m", "scene256"])
     ap.add_argument("--grid", type=int, nargs=2, default=None, metavar=("NU", "NV"),
                     help="torus300m: torus grid (default 15000 10000 = 300M triangles)")
     ap.add_argument("--resolution", type=int, nargs=2, default=None, metavar=("W", "H"),
@@ -586,6 +586,106 @@ def mesh_set_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def scene_bench(a):
+    """A scene of 256 objects, more than a frame's object table in LDS holds, so
+    frames run through the top-level BVH over the objects: four seeded displaced
+    tori of 20,000 triangles (100 x 100 grid) repeated in seeded rigid poses in
+    a cube of side 16 (5.12M triangles in all), one orbit camera at distance
+    30, 1920x1080 primary + directional shadow rays, edges + matcap.  Parity: a
+    256 x 256 crop of the G-buffer against the oracle, which loops over all 256
+    objects (no top level)."""
+    import copy
+    import j3d_amd as J
+    from j3d_amd import ffi
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from scenes import compare_gbuffers
+    rng = np.random.default_rng(5)
+    bases = [J.torus(100, 100, seed=s) for s in range(4)]
+    poses = []
+    for i in range(256):
+        q, rr = np.linalg.qr(rng.normal(size=(3, 3)))
+        q = q * np.sign(np.diag(rr))
+        if np.linalg.det(q) < 0:
+            q[:, 0] = -q[:, 0]
+        m = np.eye(4)
+        m[:3, :3] = q
+        m[:3, 3] = rng.uniform(-8.0, 8.0, 3)
+        poses.append(np.ascontiguousarray(m.T, np.float32).reshape(16))
+    r = J.Renderer(0)
+    t0 = time.time()
+    slots = [r.upload_mesh(bases[i % 4][0], bases[i % 4][1], db_id=J.make_db_id(1, i), cs=poses[i],
+                           builder=BUILDERS[a.builder]) for i in range(256)]
+    log("256 objects, %d triangles, uploaded in %.1fs" % (256 * len(bases[0][1]), time.time() - t0))
+    mc, cav = J.make_matcap(0)
+    r.set_matcap(mc, cav)
+    w, h = 1920, 1080
+    cs, ci = J.orbit_camera(30.0, 0.7, 0.45)
+    cam = J.camera_with_vertical_fov(60.0)
+    light = J.random_hemisphere(12)
+    flags = ffi.EDGES | ffi.SHADING | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    fr = J.make_frame(w, h, cam, cs, ci, np.zeros(3, np.float32), 18.0, slots, flags=flags, light_dir=light)
+    tg = r.create_target(w, h)
+    r.set_counting(True)
+    g, _ = r.render(fr, tg)
+    st = r.stats()
+    r.set_counting(False)
+    rays = int(st["primary_rays"] + st["shadow_rays"])
+    for _ in range(a.warmup):
+        r.render(fr, tg, readback=False)
+    r.synchronize()
+    r.reset_timings()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render(fr, tg, readback=False)
+    r.synchronize()
+    wall = time.perf_counter() - t1
+    tm = {k: float(np.mean(v)) for k, v in r.timings().items()}
+    value = rays * a.steps / wall / 1e6
+    hit = g["object_id"] != 0xFFFFFFFF
+    log("scene: %.1f Mrays/s, %.3f ms per frame (trace %.3f ms), %d rays, %d hit pixels on %d objects" % (
+        value, wall / a.steps * 1e3, tm["trace_ms"], rays, int(hit.sum()), len(np.unique(g["db_id"][hit]))))
+    parity, cpu = None, None
+    if not a.no_cpu_baseline and not a.profile_only:
+        import oracle as O
+        base_m = [O.Mesh(v, t) for v, t in bases]
+        inst = []
+        for i in range(256):  # instances share their base mesh's oracle BVH
+            m = copy.copy(base_m[i % 4])
+            m.cs = poses[i]
+            m.db_id = J.make_db_id(1, i)
+            inst.append(m)
+        rect = pick_crop(g, 256)
+        frc = J.make_frame(w, h, cam, cs, ci, np.zeros(3, np.float32), 18.0, slots, flags=flags, light_dir=light,
+                           rect=rect)
+        og = np.zeros((h, w), O.PIXEL_DTYPE)
+        og["object_id"] = 0xFFFFFFFF
+        t2 = time.time()
+        og, ost = O.update_canvas(inst, frc, gbuf=og)
+        cpu_s = time.time() - t2
+        x0, y0, x1, y1 = rect
+        mm = compare_gbuffers(g[y0:y1 + 1, x0:x1 + 1], og[y0:y1 + 1, x0:x1 + 1])
+        parity = {"crop": list(rect), "gbuffer_mismatches": mm}
+        crays = int(ost["primary_rays"] + ost["shadow_rays"])
+        cpu = {"value": crays / cpu_s / 1e6, "unit": "Mrays/s", "cores": os.cpu_count() or 1, "kind": "port",
+               "sample": "oracle update_canvas on the 256x256 crop over all 256 objects (%d rays, %.1fs)" % (crays, cpu_s)}
+        for m in inst:  # the copies do not own the shared BVHs
+            m.bvh = None
+        log("parity on crop %s: %s; cpu %.2f Mrays/s" % (rect, mm, cpu["value"]))
+    out = {"metric": "Mrays/s (256-object scene through the top-level BVH, 1080p primary + shadow rays)",
+           "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "256 objects: 4 displaced tori of 20,000 triangles in seeded rigid poses; "
+                                  "1920x1080 primary + directional shadow rays, edges + matcap",
+                      "objects": 256, "triangles": 256 * int(len(bases[0][1])), "builder": a.builder},
+           "trace_ms": tm["trace_ms"], "primary_node_visits_per_ray": st["node_visits"] / max(1, st["primary_rays"]),
+           "shadow_node_visits_per_ray": st["shadow_node_visits"] / max(1, st["shadow_rays"]),
+           "parity": parity, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def voxel_bench(a):
     """write_vox's voxel grid (j3d/vox.cpp:278-376) of the 28,000,000-triangle
     torus at max_dim 512 (--vox-dim): rays along x, y and z through every cell
@@ -656,6 +756,8 @@ def main():
         return mesh_set_bench(a)
     if a.workload == "vox28m":
         return voxel_bench(a)
+    if a.workload == "scene256":
+        return scene_bench(a)
     import j3d_amd as J
     from j3d_amd import ffi
     v, t, cam, cs, ci, pivot, diag, light, w, h, flags, desc = workload(a.workload)

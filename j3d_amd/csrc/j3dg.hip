@@ -190,6 +190,16 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   }
   const DevObject* const tab = kLdsObjects ? s_objs : objs;
   const TopNode* const top = kTop ? static_cast<const TopNode*>(f.top_nodes) : nullptr;
+  // a small top level is copied into LDS after the stack (all threads reach the barrier)
+  lds_top* top_l = nullptr;
+  if (kTop && f.top_lds) {
+    const uint32_t words = f.top_count * (uint32_t)(sizeof(TopNode) / 4);
+    const uint32_t* src = static_cast<const uint32_t*>(f.top_nodes);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lds_stack + D * blockDim.x);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = gload(src + i);
+    __syncthreads();
+    top_l = (lds_top*)(lds_stack + D * blockDim.x);
+  }
   Stack<D> st;
   st.init(lds_stack, spill, spill_cap);
   Counters c = {0, 0, 0}, cs = {0, 0, 0};  // primary (or query) / shadow-phase work
@@ -236,7 +246,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
       s.thi = s.tmax * kSlackUp;
       s.base = 0;
       s.cur = 0;  // the top level's root; kNone after this if no object is entered
-      top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
+      if (top_l)
+        top_next<D>(s, st, lobjs, top_l, f.top_delta, wo, wd);
+      else
+        top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
     } else {
       begin_object(s, lobjs, 0, wo, wd, tn, tf, best);
     }
@@ -246,7 +259,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   auto resume_top = [&](bool done) {
     if (!kTop || !done || (s.any && best.found)) return done;
     s.cur = kNone;
-    return top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
+    return top_l ? top_next<D>(s, st, lobjs, top_l, f.top_delta, wo, wd)
+                 : top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
   };
 
   for (;;) {
@@ -1102,6 +1116,7 @@ int Ctx::build_top_level(const uint32_t* slots, uint32_t n) {
   HIP_TRY(hipStreamSynchronize(stream), J3DG_ERR_LAUNCH);
   HIP_TRY(hipMemcpy(d_top, nodes.data(), nodes.size() * sizeof(TopNode), hipMemcpyHostToDevice), J3DG_ERR_LAUNCH);
   top_depth = depth;
+  top_count = (uint32_t)nodes.size();
   top_radius = radius;
   top_key = std::move(key);
   return J3DG_OK;
@@ -1157,9 +1172,10 @@ int Ctx::upload_object_table(const DevObject* tab, uint32_t n) {
   return J3DG_OK;
 }
 
-uint32_t Ctx::trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const {
+uint32_t Ctx::trace_grid(uint32_t n_rays, uint32_t D, uint32_t block, uint32_t extra_lds) const {
   const uint32_t waves_per_block = block / 64;
-  const uint32_t lds = D * block * sizeof(uint2) + J3DG_MAX_OBJECTS * sizeof(DevObject);  // stack + object table
+  // stack + object table (or a top level's copy)
+  const uint32_t lds = D * block * sizeof(uint2) + J3DG_MAX_OBJECTS * sizeof(DevObject) + extra_lds;
   const uint32_t rays_per_block = 64u * waves_per_block;
   uint32_t blocks_needed = (n_rays + rays_per_block - 1) / rays_per_block;
   // persistent: as many resident blocks as LDS and the 32 waves per CU allow
@@ -1172,8 +1188,10 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
   Ctx* ctx = this;
   const uint32_t D = tuning.stack_depth;
   const uint32_t block = tuning.block_size;
-  const uint32_t lds = D * block * sizeof(uint2);
-  uint32_t grid = trace_grid(n_rays, D, block);
+  const bool top_lds = (mode & kModeTop) && fp.top_count && fp.top_count * sizeof(TopNode) <= kTopLdsBytes;
+  const uint32_t top_bytes = top_lds ? fp.top_count * (uint32_t)sizeof(TopNode) : 0u;
+  const uint32_t lds = D * block * sizeof(uint2) + top_bytes;
+  uint32_t grid = trace_grid(n_rays, D, block, top_bytes);
   uint64_t lanes = (uint64_t)grid * block;
   uint32_t cap = max_stack > D ? max_stack - D + 1 : 1;
   int rc = ensure_spill(lanes, cap);
@@ -1183,6 +1201,7 @@ int Ctx::launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q,
   uint32_t* qu = qcounter;
   uint64_t* cnt = d_counters;
   FrameParams f = fp;
+  f.top_lds = top_lds ? 1u : 0u;
   RayQuery rq = q;
   uint32_t nr = n_rays, no = n_obj;
   uint32_t batch = 64u * tuning.rays_per_item;
@@ -1984,6 +2003,7 @@ int j3dg_render(j3dg_ctx* h, const j3dg_frame_desc* fd, uint32_t target, uint32_
           const float cam = std::max({std::fabs(fp.cs[12]), std::fabs(fp.cs[13]), std::fabs(fp.cs[14])});
           fp.top_nodes = ctx->d_top;
           fp.top_delta = ctx->top_delta(cam);
+          fp.top_count = ctx->top_count;
         }
       }
       if ((fd->flags & J3DG_LEARNED) && fp.n_obj) {
@@ -2076,6 +2096,7 @@ static int trace_rays_impl(Ctx* ctx, const uint32_t* slots, uint32_t n_slots, co
     }
     fp.top_nodes = ctx->d_top;
     fp.top_delta = ctx->top_delta(r);
+    fp.top_count = ctx->top_count;
     mode |= kModeTop;
   }
   int rc = ctx->launch_trace_kernel(mode, fp, q, (uint32_t)n, n_slots, max_stack, width, ctx->d_queue);

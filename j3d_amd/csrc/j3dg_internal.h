@@ -55,7 +55,11 @@ struct FrameParams {
   float margin;
   const void* top_nodes;    // kModeTop: the objects' top-level BVH (TopNode[], root first)
   float top_delta;          // kModeTop: world-box widening for the top level's box tests
+  uint32_t top_count;       // kModeTop: nodes of the top level
+  uint32_t top_lds;         // kModeTop: blocks copy the top level into LDS after the stack
 };
+// Largest top level a block copies into LDS (256 nodes of 64 bytes).
+constexpr uint32_t kTopLdsBytes = 16384;
 
 // kModeFrameColor: kModeFrame for frames with surface colours (the hit's
 // object index goes to the db_id word for k_color).
@@ -225,6 +229,7 @@ struct Ctx {
   void* d_top = nullptr;
   uint32_t top_capacity = 0;     // TopNode slots allocated
   uint32_t top_depth = 0;        // internal levels: stack entries the top level may hold
+  uint32_t top_count = 0;        // nodes of the top level
   float top_radius = 0.f;        // largest |coordinate| of the world boxes, translations and local boxes
   std::vector<float> top_key;
   // device buffers of the all-hits / voxel calls, kept between calls (grown on demand)
@@ -267,7 +272,7 @@ struct Ctx {
   uint64_t* d_timeline = nullptr;   // counting mode: per-wave timeline of the primary trace
   uint64_t timeline_capacity = 0;
   uint32_t timeline_waves = 0;
-  uint32_t trace_grid(uint32_t n_rays, uint32_t D, uint32_t block) const;
+  uint32_t trace_grid(uint32_t n_rays, uint32_t D, uint32_t block, uint32_t extra_lds = 0) const;
   int launch_shade(const ShadeParams& sp);
   // batches (j3dg_render_batch): device tables, and the captured graph of
   // reset + trace + shade replayed while its launch parameters are unchanged

@@ -675,16 +675,25 @@ struct TopNode {
   uint32_t pad[2];
 };
 
+// The top level is read from global memory, or from a block's LDS copy when
+// it is small enough (both through typed pointers: a generic pointer would
+// compile to flat loads).
+typedef const __attribute__((address_space(3))) TopNode lds_top;
+__device__ inline float top_box(const TopNode* n, int k, int a) { return gload(&n->box[k][a]); }
+__device__ inline float top_box(lds_top* n, int k, int a) { return n->box[k][a]; }
+__device__ inline uint32_t top_child(const TopNode* n, int k) { return gload(&n->child[k]); }
+__device__ inline uint32_t top_child(lds_top* n, int k) { return n->child[k]; }
+
 // Entry distance of the world ray into child k's box widened by delta, or
 // INFINITY when it misses [tlo, thi].
-__device__ inline float top_entry(const TopNode* __restrict__ n, int k, const Ray& w, float delta, float tlo,
-                                  float thi) {
+template <class NP>
+__device__ inline float top_entry(NP n, int k, const Ray& w, float delta, float tlo, float thi) {
   const float o[3] = {w.ox, w.oy, w.oz}, inv[3] = {w.ix, w.iy, w.iz};
   float en = tlo, ex = thi;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float p = (gload(&n->box[k][a]) - delta - o[a]) * inv[a];
-    const float q = (gload(&n->box[k][3 + a]) + delta - o[a]) * inv[a];
+    const float p = (top_box(n, k, a) - delta - o[a]) * inv[a];
+    const float q = (top_box(n, k, 3 + a) + delta - o[a]) * inv[a];
     en = fmaxf(en, fminf(p, q));
     ex = fminf(ex, fmaxf(p, q) * kSlackUp);
   }
@@ -706,9 +715,9 @@ __device__ inline Ray world_ray(f4 o, f4 d, float tn, float tf) {
 // bound, and starts that object's traversal (s.cur = its root).  Returns true
 // when no object is left: the ray is done.  The lane's stack holds top-level
 // entries only when this is called.
-template <int D>
-__device__ inline bool top_next(Trav& s, Stack<D>& st, const DevObject* __restrict__ objs,
-                                const TopNode* __restrict__ top, float delta, f4 wo, f4 wd) {
+template <int D, class NP>
+__device__ inline bool top_next(Trav& s, Stack<D>& st, const DevObject* __restrict__ objs, NP top, float delta,
+                                f4 wo, f4 wd) {
   if (!ray_finite(wo, wd)) {
     s.cur = kNone;
     return true;
@@ -751,8 +760,8 @@ __device__ inline bool top_next(Trav& s, Stack<D>& st, const DevObject* __restri
       s.cur = 0;  // the object's root, an internal node
       return false;
     }
-    const TopNode* n = top + cur;
-    const uint32_t c0 = gload(&n->child[0]), c1 = gload(&n->child[1]);
+    const NP n = top + cur;
+    const uint32_t c0 = top_child(n, 0), c1 = top_child(n, 1);
     float e0 = c0 != kNone ? top_entry(n, 0, w, delta, tlo, s.thi) : INFINITY;
     float e1 = c1 != kNone ? top_entry(n, 1, w, delta, tlo, s.thi) : INFINITY;
     uint32_t r0 = c0, r1 = c1;
