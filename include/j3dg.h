@@ -320,6 +320,26 @@ int j3dg_all_hits(j3dg_ctx* ctx, uint32_t slot, const float* rays, uint64_t n, u
    With data NULL only dim is returned.  Float-to-integer casts saturate into range (NaN -> 0). */
 int j3dg_voxelize(j3dg_ctx* ctx, uint32_t slot, uint32_t max_dim, uint32_t dim[3], uint8_t* data, uint64_t capacity);
 
+/* ---- point clouds (canvas::render_pointclouds_on_image, j3d/canvas.cpp:952-1030) ----
+   A cloud (scene_pointcloud, j3d/scene.h:27-38): n_points xyz, optional per-point normals (3 floats)
+   and colours (0xAABBGGRR), db_id and pose cs (NULL = identity), copied to the device; at most
+   2^32 - 2 points. */
+int j3dg_upload_points(j3dg_ctx* ctx, uint32_t db_id, const float* xyz, uint64_t n_points, const float* normals,
+                       const uint32_t* colors, const float cs[16], uint32_t* cloud);
+int j3dg_remove_points(j3dg_ctx* ctx, uint32_t cloud);
+int j3dg_set_points_cs(j3dg_ctx* ctx, uint32_t cloud, const float cs[16]);
+/* Draws the clouds, in the order given, over a target holding a rendered G-buffer and image
+   (after J3DG_STAGE_SHADE, as view::render_scene calls it, j3d/view.cpp:428).  proj = the canvas's
+   projection matrix (canvas::projection_matrix, camera.cpp:32-73), cs_inv = scene CS^-1.  Each
+   point goes to one pixel; it is drawn where its 1/w beats the z-buffer (1/depth on pixels with
+   db_id != 0, else 0; canvas.cpp:963-970) and then sets object_id = its index, depth = 1/(1/w),
+   db_id = the cloud's, and the image pixel to its colour: the cloud's, or 0xffffffff without colours
+   or with J3DG_ONE_BIT; with J3DG_SHADING and normals, R, G, B scaled by the camera-space normal's z
+   over its length, clamped to [0, 1].  On equal 1/w the earlier point (or cloud, or the surface)
+   stays.  Projection and shading restate jtk's point renderer, which the reference does not ship. */
+int j3dg_splat_points(j3dg_ctx* ctx, uint32_t target, const float proj[16], const float cs_inv[16], uint32_t flags,
+                      const uint32_t* clouds, uint32_t n_clouds);
+
 /* ---- measurement ---------------------------------------------------------
    A streaming device-to-device copy of `bytes` (read bytes, write bytes),
    repeated `repeats` times: the reference point that scales profiler byte

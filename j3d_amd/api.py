@@ -560,3 +560,33 @@ class Renderer:
                 "j3dg_voxelize")
         dims = tuple(int(x) for x in dim)
         return dims, data.reshape(dims[2], dims[1], dims[0])
+
+    def upload_points(self, xyz, normals=None, colors=None, db_id=0, cs=None):
+        """A point cloud (j3dg_upload_points): n x 3 points, optional n x 3
+        normals and n uint32 colours (0xAABBGGRR); returns its cloud id."""
+        p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        col = None if colors is None else np.ascontiguousarray(colors, np.uint32).reshape(-1)
+        assert nrm is None or nrm.shape[0] == p.shape[0]
+        assert col is None or col.shape[0] == p.shape[0]
+        csa = None if cs is None else np.ascontiguousarray(cs, np.float32)
+        cloud = C.c_uint32(0)
+        self._c(self.lib.j3dg_upload_points(self.ctx, int(db_id), ffi.ptr(p), p.shape[0], ffi.ptr(nrm), ffi.ptr(col),
+                                            ffi.ptr(csa), C.byref(cloud)), "j3dg_upload_points")
+        return int(cloud.value)
+
+    def remove_points(self, cloud):
+        self._c(self.lib.j3dg_remove_points(self.ctx, int(cloud)), "j3dg_remove_points")
+
+    def set_points_cs(self, cloud, cs):
+        a = np.ascontiguousarray(cs, np.float32)
+        self._c(self.lib.j3dg_set_points_cs(self.ctx, int(cloud), ffi.ptr(a)), "j3dg_set_points_cs")
+
+    def splat_points(self, target, proj, cs_inv, flags, clouds):
+        """Draws point clouds over a rendered target (j3dg_splat_points,
+        canvas::render_pointclouds_on_image); read the result with read_target."""
+        P = np.ascontiguousarray(proj, np.float32)
+        ci = np.ascontiguousarray(cs_inv, np.float32)
+        lst = np.ascontiguousarray(list(clouds), np.uint32)
+        self._c(self.lib.j3dg_splat_points(self.ctx, int(target), ffi.ptr(P), ffi.ptr(ci), int(flags), ffi.ptr(lst),
+                                           len(lst)), "j3dg_splat_points")

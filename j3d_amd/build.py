@@ -29,7 +29,7 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-flush-denormals-to
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-value"]
 
-HIP_SOURCES = ["j3dg.hip", "mlp.hip", "batch.hip", "bvh_gpu.hip", "allhits.hip"]
+HIP_SOURCES = ["j3dg.hip", "mlp.hip", "batch.hip", "bvh_gpu.hip", "allhits.hip", "splat.hip"]
 CPP_SOURCES = ["bvh_build.cpp", "scenegen.cpp", "host_tables.cpp", "autotune.cpp"]
 
 

@@ -1497,6 +1497,7 @@ void j3dg_destroy(j3dg_ctx* h) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (auto& m : ctx->meshes) m.release();
   for (auto& t : ctx->targets) t.release();
+  for (auto& c : ctx->clouds) c.release();
   ctx->release_buffers();
   for (auto& e : ctx->ev)
     if (e) hipEventDestroy(e);

@@ -185,6 +185,26 @@ struct Target {
   }
 };
 
+// A point cloud (scene_pointcloud, j3d/scene.h:27-38) on the device: points,
+// optional normals and colours (0xAABBGGRR), and its pose.
+struct PointCloud {
+  bool live = false;
+  uint32_t db_id = 0;
+  uint64_t n = 0;
+  float cs[16];
+  float* d_xyz = nullptr;
+  float* d_normals = nullptr;
+  uint32_t* d_colors = nullptr;
+  void release() {
+    if (d_xyz) hipFree(d_xyz);
+    if (d_normals) hipFree(d_normals);
+    if (d_colors) hipFree(d_colors);
+    d_xyz = nullptr, d_normals = nullptr, d_colors = nullptr;
+    n = 0;
+    live = false;
+  }
+};
+
 struct Ctx {
   std::mutex mu;
   int device = 0;
@@ -206,6 +226,7 @@ struct Ctx {
   j3dg_tuning tuning{16, 1, 1, 256, 32, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
+  std::vector<PointCloud> clouds;
   uint32_t* d_queue = nullptr;
   uint64_t* d_counters = nullptr;
   uint2* spill = nullptr;

@@ -48,6 +48,50 @@ J3D_HD uint32_t color_byte(float c) {
   return (uint32_t)(int32_t)x & 0xFFu;
 }
 
+// Point splats (canvas::render_pointclouds_on_image, j3d/canvas.cpp:952-1030).
+// The reference binds the scene's CS^-1, the cloud's cs and the canvas
+// projection P to jtk's point renderer (absent from the reference tree) and
+// keeps a z-buffer of 1/depth.  Defined here: a point p is drawn into one
+// pixel, clip = P (CS^-1 (cs (p, 1))), each product four terms left to right;
+// it is visible when 0 < clip.w <= FLT_MAX and |x|, |y| <= w (NaN fails).  There is no
+// near/far test on clip z: with the reference's default far plane (FLT_MAX,
+// j3d/camera.cpp:12) the frustum's z row holds -inf (2 * far overflows,
+// camera.cpp:18-24), so such a test would drop every point; depth is judged
+// by 1/w alone.  Its pixel is floor((x/w + 1) * 0.5 * W) and
+// floor((y/w + 1) * 0.5 * H), rejected when equal to W or H, so that a point on
+// the ray-cast surface of pixel (x, y) (canvas.cpp:773) lands in (x, y); its
+// z-buffer value is 1/w.
+J3D_HD bool splat_project(const float* cs, const float* cs_inv, const float* proj, float px, float py, float pz,
+                          uint32_t w, uint32_t h, uint32_t& pixel, float& inv_w) {
+  const f4 c = mat_vec(proj, mat_vec(cs_inv, mat_vec(cs, f4{px, py, pz, 1.f})));
+  if (!(c.w > 0.f) || !(c.w <= FLT_MAX) || !(fabsf(c.x) <= c.w) || !(fabsf(c.y) <= c.w)) return false;
+  const float fx = (c.x / c.w + 1.f) * 0.5f * (float)w;
+  const float fy = (c.y / c.w + 1.f) * 0.5f * (float)h;
+  const uint32_t x = (uint32_t)fx, y = (uint32_t)fy;  // fx, fy in [0, W], [0, H]
+  if (x >= w || y >= h) return false;
+  pixel = y * w + x;
+  inv_w = 1.f / c.w;
+  return true;
+}
+
+// The colour a drawn point writes: its own colour (0xAABBGGRR), or 0xffffffff
+// (the colour argument of the reference's present call, canvas.cpp:997) when
+// the cloud has none or the canvas is one-bit; with shading and normals the
+// three colour channels are scaled by l = n'.z / |n'|, n' = CS^-1 (cs (n, 0)),
+// clamped to [0, 1] (NaN -> 0), each truncated to an integer.
+J3D_HD uint32_t splat_color(uint32_t color, bool shade, const float* cs, const float* cs_inv, float nx, float ny,
+                            float nz) {
+  if (!shade) return color;
+  const f4 n = mat_vec(cs_inv, mat_vec(cs, f4{nx, ny, nz, 0.f}));
+  float l = n.z / sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+  if (!(l > 0.f)) l = 0.f;
+  if (l > 1.f) l = 1.f;
+  const uint32_t r = (uint32_t)((float)(color & 0xFFu) * l);
+  const uint32_t g = (uint32_t)((float)((color >> 8) & 0xFFu) * l);
+  const uint32_t b = (uint32_t)((float)((color >> 16) & 0xFFu) * l);
+  return (color & 0xff000000u) | (b << 16) | (g << 8) | r;
+}
+
 // Moller-Trumbore ray/triangle test on (v0, e1 = v1-v0, e2 = v2-v0): two-sided,
 // det == 0 rejects, barycentric and [t_near, t_far] tests inclusive, NaN rejects.
 // This defines the test behind jtk's find_closest_triangle (j3d/canvas.cpp:786).

@@ -136,6 +136,10 @@ SIGNATURES = {
     "j3dg_upload_mesh": (_I, [_P, _U32, _P, _U64, _P, _U64, _P, C.POINTER(BuildOpts), C.POINTER(_U32)]),
     "j3dg_all_hits": (_I, [_P, _U32, _P, _U64, _U32, _P, _P, _P, _P, _P]),
     "j3dg_voxelize": (_I, [_P, _U32, _U32, _P, _P, _U64]),
+    "j3dg_upload_points": (_I, [_P, _U32, _P, _U64, _P, _P, _P, _P]),
+    "j3dg_remove_points": (_I, [_P, _U32]),
+    "j3dg_set_points_cs": (_I, [_P, _U32, _P]),
+    "j3dg_splat_points": (_I, [_P, _U32, _P, _P, _U32, _P, _U32]),
     "j3dg_remove_mesh": (_I, [_P, _U32]),
     "j3dg_set_mesh_cs": (_I, [_P, _U32, _P]),
     "j3dg_set_mesh_colors": (_I, [_P, _U32, _P, _P, _P, _U32, _U32]),

@@ -1255,4 +1255,56 @@ void orc_canvas_to_image(const orc_pixel* gbuf, uint32_t w, uint32_t h, const fl
     }
 }
 
+
+/* ---- point clouds: canvas::render_pointclouds_on_image (j3d/canvas.cpp:952-1030) ---- */
+
+/* The z-buffer of canvas.cpp:963-970: 1/depth where db_id != 0, else 0. */
+void orc_splat_zbuffer(const orc_pixel* gbuf, uint64_t n_pix, float* z) {
+  for (uint64_t i = 0; i < n_pix; ++i) z[i] = gbuf[i].db_id != 0 ? 1.f / gbuf[i].depth : 0.f;
+}
+
+/* One cloud drawn point after point in index order (canvas.cpp:983-1026) into
+   one pixel each.  jtk's point renderer is absent from the reference tree; its
+   projection is restated as clip = P (CS^-1 (cs p)), drawn when w is positive
+   and finite and -w <= x, y <= w (no test on clip z, which the default far plane FLT_MAX of
+   j3d/camera.cpp:12 makes infinite), at pixel floor((x/w + 1) * 0.5 * W), floor((y/w + 1) *
+   0.5 * H) (W and H themselves dropped), depth test 1/w > z (strict).  A drawn
+   point writes object_id = i, depth = 1/z, db_id (canvas.cpp:1000-1026) and
+   its colour (0xffffffff without colours or in one-bit mode, canvas.cpp:995,
+   997), with shading and normals each of R, G, B times n'.z / |n'| clamped to
+   [0, 1], n' = CS^-1 (cs (n, 0)), truncated. */
+void orc_splat_cloud(orc_pixel* gbuf, uint32_t* im, uint32_t w, uint32_t h, float* z, const float* proj,
+                     const float* cs_inv, const float* cs, uint32_t db_id, const float* xyz, uint64_t n,
+                     const float* normals, const uint32_t* colors, int shading, int one_bit) {
+  const float* nrm = shading ? normals : nullptr;
+  const uint32_t* col = one_bit ? nullptr : colors;
+  for (uint64_t i = 0; i < n; ++i) {
+    const f4 c = mv(proj, mv(cs_inv, mv(cs, f4{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.f})));
+    if (!(c.w > 0.f) || std::isinf(c.w)) continue;
+    if (!(c.x >= -c.w && c.x <= c.w && c.y >= -c.w && c.y <= c.w)) continue;
+    const float sx = (c.x / c.w + 1.f) * 0.5f * (float)w;
+    const float sy = (c.y / c.w + 1.f) * 0.5f * (float)h;
+    const uint32_t px = (uint32_t)sx, py = (uint32_t)sy;
+    if (px >= w || py >= h) continue;
+    const size_t k = (size_t)py * w + px;
+    const float iw = 1.f / c.w;
+    if (!(iw > z[k])) continue;
+    z[k] = iw;
+    gbuf[k].object_id = (uint32_t)i;
+    gbuf[k].depth = 1.f / z[k];
+    gbuf[k].db_id = db_id;
+    uint32_t clr = col ? col[i] : 0xffffffffu;
+    if (nrm) {
+      const f4 nn = mv(cs_inv, mv(cs, f4{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.f}));
+      const float len = std::sqrt(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+      float l = nn.z / len;
+      l = l > 0.f ? (l < 1.f ? l : 1.f) : 0.f;
+      uint32_t ch[3];
+      for (int j = 0; j < 3; ++j) ch[j] = (uint32_t)((float)((clr >> (8 * j)) & 0xFFu) * l);
+      clr = (clr & 0xff000000u) | (ch[2] << 16) | (ch[1] << 8) | ch[0];
+    }
+    im[k] = clr;
+  }
+}
+
 }  // extern "C"

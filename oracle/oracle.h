@@ -170,6 +170,14 @@ void orc_voxelize(const orc_object* ob, const float bb_min[3], const float bb_ma
 /* vox.cpp:154-171 */
 uint8_t orc_vox_color_index(uint8_t r, uint8_t g, uint8_t b);
 
+/* Point clouds drawn over a G-buffer and image (j3d/canvas.cpp:952-1030): the z-buffer of
+   canvas.cpp:963-970, then one call per cloud in scene order, each drawing its points in index
+   order (see oracle.cpp for the restated projection and colour). */
+void orc_splat_zbuffer(const orc_pixel* gbuf, uint64_t n_pix, float* z);
+void orc_splat_cloud(orc_pixel* gbuf, uint32_t* im, uint32_t w, uint32_t h, float* z, const float* proj,
+                     const float* cs_inv, const float* cs, uint32_t db_id, const float* xyz, uint64_t n,
+                     const float* normals, const uint32_t* colors, int shading, int one_bit);
+
 /* Single ray/triangle test, exposed for known-answer tests. Returns 1 on hit. */
 int orc_intersect_triangle(const float o[3], const float d[3], const float v0[3], const float v1[3],
                            const float v2[3], float t_near, float t_far, float* t, float* u, float* v);

@@ -89,6 +89,9 @@ def load():
         "orc_vox_dims": (None, [_P, _P, C.c_uint32, _P]),
         "orc_voxelize": (None, [_P, _P, _P, _P, _P]),
         "orc_vox_color_index": (C.c_uint8, [C.c_uint8, C.c_uint8, C.c_uint8]),
+        "orc_splat_zbuffer": (None, [_P, C.c_uint64, _P]),
+        "orc_splat_cloud": (None, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, _P,
+                                   _P, C.c_int, C.c_int]),
         "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
         "orc_color_byte": (C.c_uint8, [C.c_float]),
         "orc_mlp_param_count": (C.c_int, []),
@@ -305,6 +308,34 @@ def voxelize(mesh, max_dim, brute=False):
     d = np.array(dims, np.uint32)
     load().orc_voxelize(arr, _p(lo), _p(hi), _p(d), _p(data))
     return dims, data.reshape(dims[2], dims[1], dims[0])
+
+
+def splat_points(gbuf, rgba, proj, cs_inv, flags, clouds):
+    """Point clouds drawn over a rendered G-buffer and image
+    (canvas::render_pointclouds_on_image): clouds is a list of dicts with
+    xyz (n x 3), optional normals (n x 3), colors (n uint32), db_id and cs.
+    Returns new (gbuf, rgba) arrays."""
+    g = np.ascontiguousarray(gbuf, PIXEL_DTYPE).copy()
+    im = np.ascontiguousarray(rgba, np.uint32).copy()
+    h, w = g.shape
+    if not clouds:
+        return g, im
+    z = np.zeros(h * w, np.float32)
+    lib = load()
+    lib.orc_splat_zbuffer(_p(g), h * w, _p(z))
+    P = np.ascontiguousarray(proj, np.float32)
+    ci = np.ascontiguousarray(cs_inv, np.float32)
+    for c in clouds:
+        xyz = np.ascontiguousarray(c["xyz"], np.float32).reshape(-1, 3)
+        nrm = c.get("normals")
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        col = c.get("colors")
+        col = None if col is None else np.ascontiguousarray(col, np.uint32).reshape(-1)
+        cs = c.get("cs")
+        cs = np.eye(4, dtype=np.float32).reshape(16) if cs is None else np.ascontiguousarray(cs, np.float32)
+        lib.orc_splat_cloud(_p(g), _p(im), w, h, _p(z), _p(P), _p(ci), _p(cs), int(c.get("db_id", 0)), _p(xyz),
+                            xyz.shape[0], _p(nrm), _p(col), int(bool(flags & SHADING)), int(bool(flags & ONE_BIT)))
+    return g, im
 
 
 def vox_color_index(r, g, b):
