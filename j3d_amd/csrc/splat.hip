@@ -17,7 +17,8 @@
 // what the pixel already shows beats every point), then one launch over the
 // pixels writes the winner's fields and marks the key as shown, so that the
 // next cloud's points lose ties to it.  The result does not depend on the
-// order in which the points are processed.
+// order in which the points are processed; a point first reads its pixel's
+// key and skips the atomic when the key is already at least its own.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -69,7 +70,12 @@ __global__ void __launch_bounds__(kSplatBlock) k_splat_depth(const float* __rest
       continue;
     const unsigned long long key =
         ((unsigned long long)order_bits(inv_w) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)i);
-    atomicMax(keys + pixel, key);
+    // Keys only grow during this launch, so any value read here (untorn: a
+    // 64-bit atomic load, possibly older than the latest) is a lower bound of
+    // the final key: when it is already >= this key the atomic cannot change
+    // anything and is skipped.  Most points lose, so most atomics go.
+    const unsigned long long seen = __hip_atomic_load(keys + pixel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (key > seen) atomicMax(keys + pixel, key);
   }
 }
 

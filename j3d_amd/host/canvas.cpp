@@ -83,6 +83,33 @@ void device::check(int rc, const char* what) const {
 scene::~scene() {
   if (!dev) return;
   for (const auto& o : objects) j3dg_remove_mesh(dev->ctx(), o.slot);
+  for (const auto& p : pointclouds) j3dg_remove_points(dev->ctx(), p.cloud);
+}
+
+void add_pointcloud(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<float>* normals,
+                    const std::vector<uint32_t>* colors, const float4x4& cs) {
+  const size_t n = vertices.size() / 3;
+  if (vertices.size() % 3 || (normals && !normals->empty() && normals->size() != vertices.size()) ||
+      (colors && !colors->empty() && colors->size() != n))
+    throw std::invalid_argument("add_pointcloud: arrays of xyz, normals and colours per point");
+  scene_pointcloud pc;
+  pc.db_id = id;
+  pc.cs = cs;
+  pc.device_cs = cs;
+  s.dev->check(j3dg_upload_points(s.dev->ctx(), id, vertices.data(), n,
+                                  normals && !normals->empty() ? normals->data() : nullptr,
+                                  colors && !colors->empty() ? colors->data() : nullptr, cs.data(), &pc.cloud),
+               "j3dg_upload_points");
+  s.pointclouds.push_back(pc);
+}
+
+void remove_pointcloud(uint32_t id, scene& s) {
+  auto it = std::find_if(s.pointclouds.begin(), s.pointclouds.end(),
+                         [&](const scene_pointcloud& p) { return p.db_id == id; });
+  if (it != s.pointclouds.end()) {
+    s.dev->check(j3dg_remove_points(s.dev->ctx(), it->cloud), "j3dg_remove_points");
+    s.pointclouds.erase(it);
+  }
 }
 
 // j3d/scene.cpp:8-25
@@ -368,6 +395,38 @@ void canvas::canvas_to_image(const image<pixel>& pixels, const matcap& _matcap) 
 }
 
 void canvas::canvas_to_image(const matcap& _matcap) { canvas_to_image(_canvas, _matcap); }
+
+// j3d/canvas.cpp:952-1030.  The device target holds the image canvas_to_image
+// made; its G-buffer is set to `pix` (the z-buffer source) unless it already
+// is.  Afterwards the canvas's pixels are `pix` with the points drawn: the
+// reference writes the drawn pixels into _canvas, and view::render_scene
+// passes a copy of _canvas as `pix` (view.cpp:426-428), so the two agree.
+void canvas::render_pointclouds_on_image(const scene* s, const image<pixel>& pix) {
+  if (!s || s->pointclouds.empty()) return;
+  if (pix.width() != w_ || pix.height() != h_)
+    throw std::invalid_argument("render_pointclouds_on_image: image size differs");
+  const bool on_device = (&pix == &_canvas && pixels_stale_) || (pix.tag() != 0 && pix.tag() == generation_);
+  if (!on_device) {
+    if (&pix != &_canvas) pull_pixels();
+    dev_->check(j3dg_write_target_gbuffer(dev_->ctx(), target_, pix.data()), "j3dg_write_target_gbuffer");
+  }
+  std::vector<uint32_t> clouds;
+  for (auto& p : const_cast<scene*>(s)->pointclouds) {
+    if (std::memcmp(p.cs.data(), p.device_cs.data(), sizeof(float) * 16) != 0) {
+      dev_->check(j3dg_set_points_cs(dev_->ctx(), p.cloud, p.cs.data()), "j3dg_set_points_cs");
+      p.device_cs = p.cs;
+    }
+    clouds.push_back(p.cloud);
+  }
+  const j3dg_frame_desc d = frame_desc(nullptr, 0, 0, (int)w_ - 1, (int)h_ - 1);
+  dev_->check(j3dg_splat_points(dev_->ctx(), target_, projection_matrix.data(), s->coordinate_system_inv.data(),
+                                d.flags, clouds.data(), (uint32_t)clouds.size()),
+              "j3dg_splat_points");
+  dev_->check(j3dg_read_target(dev_->ctx(), target_, _canvas.data(), im.data()), "j3dg_read_target");
+  generation_ = next_generation();
+  _canvas.set_tag(generation_);
+  pixels_stale_ = false;
+}
 
 // j3d/canvas.cpp:900-951: copy into `screen`, flipped vertically; pos_x must be
 // a multiple of 4 as in the reference.

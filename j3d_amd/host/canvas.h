@@ -128,6 +128,15 @@ struct scene_object {
   float4x4 device_cs = get_identity();
 };
 
+// scene_pointcloud (j3d/scene.h:27-38): points, optional normals and colours
+// (0xAABBGGRR), copied to the device by add_pointcloud.
+struct scene_pointcloud {
+  uint32_t db_id = 0;
+  float4x4 cs = get_identity();
+  uint32_t cloud = 0;  // device cloud id
+  float4x4 device_cs = get_identity();
+};
+
 // scene (j3d/scene.h:40-56)
 struct scene {
   explicit scene(std::shared_ptr<device> d) : dev(std::move(d)) {}
@@ -138,6 +147,7 @@ struct scene {
   float min_bb[3] = {0, 0, 0}, max_bb[3] = {0, 0, 0};
   float diagonal = 0.f;
   std::list<scene_object> objects;
+  std::list<scene_pointcloud> pointclouds;
 };
 
 // add_object (j3d/scene.cpp:8-39) for a mesh held by the caller: normals, bounding
@@ -154,6 +164,11 @@ void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const
 // during the call.
 void set_object_colors(uint32_t id, scene& s, const std::vector<float>* vertex_colors,
                        const std::vector<float>* uv = nullptr, const image<uint32_t>* texture = nullptr);
+// A point cloud of the scene (scene::pointclouds, j3d/scene.h:50): 3 floats per
+// point; normals (3 floats per point) and colours may be null or empty.
+void add_pointcloud(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<float>* normals,
+                    const std::vector<uint32_t>* colors, const float4x4& cs = get_identity());
+void remove_pointcloud(uint32_t id, scene& s);
 // remove_object (j3d/scene.cpp:41-49)
 void remove_object(uint32_t id, scene& s);
 // prepare_scene / unzoom (j3d/scene.cpp:50-110)
@@ -192,6 +207,9 @@ class canvas {
 
   void canvas_to_image(const image<pixel>& canvas, const matcap& _matcap);
   void canvas_to_image(const matcap& _matcap);
+  // j3d/canvas.cpp:952-1030: the scene's point clouds drawn into the image and
+  // the G-buffer, under a z-buffer from `pix` (see j3dg_splat_points).
+  void render_pointclouds_on_image(const scene* s, const image<pixel>& pix);
 
   const float4x4& get_projection_matrix() const { return projection_matrix; }
   const float4x4& get_inverse_projection_matrix() const { return projection_matrix_inv; }

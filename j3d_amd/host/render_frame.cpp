@@ -5,8 +5,9 @@
 //   j3d_render_frame --scene icosphere LEVEL SEED AMP | --scene torus NU NV SEED
 //                    --size W H [--focal MM] [--cs 16 floats] [--flags edges,shadow,...]
 //                    [--light X Y Z] [--matcap 0..3] [--bg TOP BOTTOM] [--rect X0 Y0 X1 Y1]
-//                    [--frames N] [--learned MARGIN] [--copies N SPACING] [--device-build] [--out PREFIX]
+//                    [--frames N] [--learned MARGIN] [--copies N SPACING] [--points N SEED] [--device-build] [--out PREFIX]
 // --copies: N objects of the mesh, object k translated by k * SPACING along x
+// --points N SEED: a point cloud of N points near the mesh, drawn after the shading
 // (db_id 2^29 + k).
 // Prints one JSON line (sizes, times).  Without --cs the scene is framed by
 // prepare_scene + unzoom (j3d/scene.cpp:50-110).
@@ -52,7 +53,8 @@ bool write_file(const std::string& path, const void* data, size_t bytes) {
 
 int main(int argc, char** argv) {
   std::string kind, flags = "edges,shading", out;
-  uint32_t level = 7, nu = 0, nv = 0, w = 0, h = 0, frames = 1, mc = 0, copies = 1;
+  uint32_t level = 7, nu = 0, nv = 0, w = 0, h = 0, frames = 1, mc = 0, copies = 1, n_points = 0;
+  uint64_t point_seed = 0;
   float spacing = 0.f;
   uint32_t builder = J3DG_BUILD_HOST;
   uint64_t seed = 1;
@@ -125,6 +127,10 @@ int main(int argc, char** argv) {
       copies = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
       spacing = std::strtof(argv[++i], nullptr);
       if (!copies) usage("--copies needs at least one object");
+    } else if (a == "--points") {
+      need(2);
+      n_points = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
+      point_seed = std::strtoull(argv[++i], nullptr, 10);
     } else if (a == "--device-build") {
       builder = J3DG_BUILD_DEVICE;
     } else if (a == "--out") {
@@ -156,6 +162,34 @@ int main(int argc, char** argv) {
       float4x4 ocs = get_identity();
       ocs[12] = (float)k * spacing;
       add_object((1u << 29) + k, sc, xyz, tri, ocs, 8, builder);
+    }
+    // --points: one point cloud of mesh vertices (cycled) pushed off the surface by
+    // an LCG of the seed, with normals and colours; written to PREFIX.points
+    std::vector<float> pxyz, pnrm;
+    std::vector<uint32_t> pcol;
+    if (n_points) {
+      uint64_t r = point_seed * 6364136223846793005ull + 1442695040888963407ull;
+      auto next = [&]() {
+        r = r * 6364136223846793005ull + 1442695040888963407ull;
+        return (uint32_t)(r >> 33);
+      };
+      const size_t nvx = xyz.size() / 3;
+      for (uint32_t i = 0; i < n_points; ++i) {
+        const size_t j = (size_t)next() % nvx;
+        for (int k = 0; k < 3; ++k) {
+          pxyz.push_back(xyz[3 * j + k] + ((float)(next() & 0xFFFF) / 65535.f - 0.5f) * 0.04f);
+          pnrm.push_back((float)(next() & 0xFFFF) / 32767.5f - 1.f);
+        }
+        pcol.push_back(next() * 2654435761u);
+      }
+      add_pointcloud((2u << 29) + 1, sc, pxyz, &pnrm, &pcol);
+      if (!out.empty()) {
+        std::vector<float> rec(pxyz);
+        rec.insert(rec.end(), pnrm.begin(), pnrm.end());
+        if (!write_file(out + ".points", rec.data(), rec.size() * sizeof(float)) ||
+            !write_file(out + ".colors", pcol.data(), pcol.size() * sizeof(uint32_t)))
+          usage("cannot write point files");
+      }
     }
     const auto t1 = std::chrono::steady_clock::now();
     prepare_scene(sc);
@@ -198,6 +232,7 @@ int main(int argc, char** argv) {
       }
       pixels = cv.get_pixels();
       cv.canvas_to_image(pixels, m);
+      cv.render_pointclouds_on_image(&sc, pixels);
       wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     }
     float tr = 0, sh = 0, ml = 0, sd = 0;

@@ -44,7 +44,7 @@ def _oracle_frame(v, t, w, h, cam, cs, ci, flags, light, rect=None, mc_kind=0):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["full", "rect", "torus_shadow"])
+@pytest.mark.parametrize("case", ["full", "rect", "torus_shadow", "points"])
 def test_canvas_frame_matches_oracle(tmp_path, case):
     w, h = 320, 240
     cam = J.camera_with_vertical_fov(60.0)
@@ -52,7 +52,7 @@ def test_canvas_frame_matches_oracle(tmp_path, case):
     light = np.array([0.3, 0.5, 0.81], np.float32)
     light /= np.linalg.norm(light)
     rect = None
-    if case == "torus_shadow":
+    if case in ("torus_shadow", "points"):
         v, t = J.torus(300, 170, seed=2)
         cs, ci = J.orbit_camera(3.0, 0.7, 0.5)
         flags = ffi.EDGES | ffi.SHADING | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
@@ -72,12 +72,24 @@ def test_canvas_frame_matches_oracle(tmp_path, case):
         cmd += ["--light"] + [repr(float(x)) for x in light]
     if rect:
         cmd += ["--rect"] + [str(x) for x in rect]
+    if case == "points":  # render_pointclouds_on_image after canvas_to_image (view.cpp:428)
+        cmd += ["--points", "60000", "5"]
     print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     g = np.fromfile(prefix + ".gbuf", ffi.PIXEL_DTYPE).reshape(h, w)
     im = np.fromfile(prefix + ".rgba", np.uint32).reshape(h, w)
     og, oim = _oracle_frame(v, t, w, h, cam, cs, ci, flags, light, rect=rect)
+    if case == "points":
+        rec = np.fromfile(prefix + ".points", np.float32).reshape(2, -1, 3)
+        col = np.fromfile(prefix + ".colors", np.uint32)
+        P, _ = J.projection(cam, w, h)
+        # the harness's scene CS^-1 is invert_orthonormal(--cs), which can differ
+        # from orbit_camera's inverse in the translation's last bit; points see it
+        ci_scene = J.invert_orthonormal(cs)
+        og, oim = O.splat_points(og, oim, P, ci_scene, flags, [{"xyz": rec[0], "normals": rec[1], "colors": col,
+                                                           "db_id": (2 << 29) + 1}])
+        assert (g["db_id"] == (2 << 29) + 1).sum() > 1000
     assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
     assert compare_rgba(im, oim)[0] <= 1
 
