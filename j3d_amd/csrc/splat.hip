@@ -31,6 +31,7 @@ namespace j3d {
 namespace {
 
 constexpr uint32_t kSplatBlock = 256;
+constexpr int kSplatBatch = 4;  // points per thread and loop trip in k_splat_depth
 constexpr uint64_t kMaxCloudPoints = 0xFFFFFFFEull;  // point i's key holds 0xFFFFFFFE - i
 
 struct SplatMats {
@@ -62,20 +63,33 @@ __global__ void k_splat_init(const j3dg_pixel* __restrict__ gbuf, unsigned long 
 __global__ void __launch_bounds__(kSplatBlock) k_splat_depth(const float* __restrict__ xyz, uint64_t n, SplatMats m,
                                                              uint32_t w, uint32_t h,
                                                              unsigned long long* __restrict__ keys) {
+  // kSplatBatch points per thread and trip (i, i + stride, ...): their point
+  // and key reads are issued together, then the atomics, so that the reads of
+  // one trip overlap instead of waiting behind the previous point's atomic.
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t pixel;
-    float inv_w;
-    if (!splat_project(m.cs, m.cs_inv, m.proj, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w, h, pixel, inv_w))
-      continue;
-    const unsigned long long key =
-        ((unsigned long long)order_bits(inv_w) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)i);
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; base < n; base += stride * kSplatBatch) {
+    uint32_t pixel[kSplatBatch];
+    unsigned long long key[kSplatBatch], seen[kSplatBatch];
+    bool ok[kSplatBatch];
+#pragma unroll
+    for (int b = 0; b < kSplatBatch; ++b) {
+      const uint64_t i = base + (uint64_t)b * stride;
+      float inv_w = 0.f;
+      pixel[b] = 0;
+      ok[b] = i < n &&
+              splat_project(m.cs, m.cs_inv, m.proj, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w, h, pixel[b], inv_w);
+      key[b] = ((unsigned long long)order_bits(inv_w) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)i);
+    }
     // Keys only grow during this launch, so any value read here (untorn: a
     // 64-bit atomic load, possibly older than the latest) is a lower bound of
     // the final key: when it is already >= this key the atomic cannot change
     // anything and is skipped.  Most points lose, so most atomics go.
-    const unsigned long long seen = __hip_atomic_load(keys + pixel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (key > seen) atomicMax(keys + pixel, key);
+#pragma unroll
+    for (int b = 0; b < kSplatBatch; ++b)
+      seen[b] = ok[b] ? __hip_atomic_load(keys + pixel[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : ~0ull;
+#pragma unroll
+    for (int b = 0; b < kSplatBatch; ++b)
+      if (ok[b] && key[b] > seen[b]) atomicMax(keys + pixel[b], key[b]);
   }
 }
 
