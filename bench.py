@@ -184,26 +184,6 @@ def orbit_cameras(n, seed):
     return out
 
 
-def pick_crop(gbuf, side):
-    """The side x side window, on a half-overlapping grid of positions, whose
-    hit fraction is closest to 3/4: mostly surface, with some silhouette."""
-    h, w = gbuf.shape
-    side = min(side, w, h)
-    hit = (gbuf["object_id"] != 0xFFFFFFFF).astype(np.int64)
-    ii = np.zeros((h + 1, w + 1), np.int64)
-    ii[1:, 1:] = hit.cumsum(0).cumsum(1)
-    best = None
-    step = max(1, side // 2)
-    for y0 in range(0, h - side + 1, step):
-        for x0 in range(0, w - side + 1, step):
-            n = ii[y0 + side, x0 + side] - ii[y0, x0 + side] - ii[y0 + side, x0] + ii[y0, x0]
-            d = abs(n / float(side * side) - 0.75)
-            if best is None or d < best[0]:
-                best = (d, x0, y0)
-    _, x0, y0 = best
-    return (x0, y0, x0 + side - 1, y0 + side - 1)
-
-
 # j3dg_build_opts.builder values (J3DG_BUILD_HOST, J3DG_BUILD_DEVICE)
 BUILDERS = {"host": 0, "device": 1}
 
@@ -231,7 +211,7 @@ def multi_camera(a):
     from j3d_amd import ffi
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from scenes import compare_gbuffers
+    from scenes import compare_gbuffers, pick_crop
     nu, nv = a.grid or (15000, 10000)
     w, h = a.resolution or (3840, 2160)
     t0 = time.time()
@@ -599,7 +579,7 @@ def scene_bench(a):
     from j3d_amd import ffi
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from scenes import compare_gbuffers
+    from scenes import compare_gbuffers, pick_crop
     rng = np.random.default_rng(5)
     bases = [J.torus(100, 100, seed=s) for s in range(4)]
     poses = []

@@ -21,6 +21,14 @@ def make_db_id(key, index):
     return ((key << 29) | index) & 0xFFFFFFFF
 
 
+def _matrix16(m, what):
+    """A 4x4 matrix as 16 contiguous float32 values (the C side reads 16)."""
+    a = np.ascontiguousarray(m, np.float32).reshape(-1)
+    if a.size != 16:
+        raise ValueError("%s: a 4x4 matrix has 16 values, got %d" % (what, a.size))
+    return a
+
+
 class J3dgError(RuntimeError):
     pass
 
@@ -567,9 +575,11 @@ class Renderer:
         p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
         col = None if colors is None else np.ascontiguousarray(colors, np.uint32).reshape(-1)
-        assert nrm is None or nrm.shape[0] == p.shape[0]
-        assert col is None or col.shape[0] == p.shape[0]
-        csa = None if cs is None else np.ascontiguousarray(cs, np.float32)
+        if nrm is not None and nrm.shape[0] != p.shape[0]:
+            raise ValueError("upload_points: %d normals for %d points" % (nrm.shape[0], p.shape[0]))
+        if col is not None and col.shape[0] != p.shape[0]:
+            raise ValueError("upload_points: %d colours for %d points" % (col.shape[0], p.shape[0]))
+        csa = None if cs is None else _matrix16(cs, "upload_points: cs")
         cloud = C.c_uint32(0)
         self._c(self.lib.j3dg_upload_points(self.ctx, int(db_id), ffi.ptr(p), p.shape[0], ffi.ptr(nrm), ffi.ptr(col),
                                             ffi.ptr(csa), C.byref(cloud)), "j3dg_upload_points")
@@ -579,14 +589,14 @@ class Renderer:
         self._c(self.lib.j3dg_remove_points(self.ctx, int(cloud)), "j3dg_remove_points")
 
     def set_points_cs(self, cloud, cs):
-        a = np.ascontiguousarray(cs, np.float32)
+        a = _matrix16(cs, "set_points_cs: cs")
         self._c(self.lib.j3dg_set_points_cs(self.ctx, int(cloud), ffi.ptr(a)), "j3dg_set_points_cs")
 
     def splat_points(self, target, proj, cs_inv, flags, clouds):
         """Draws point clouds over a rendered target (j3dg_splat_points,
         canvas::render_pointclouds_on_image); read the result with read_target."""
-        P = np.ascontiguousarray(proj, np.float32)
-        ci = np.ascontiguousarray(cs_inv, np.float32)
+        P = _matrix16(proj, "splat_points: proj")
+        ci = _matrix16(cs_inv, "splat_points: cs_inv")
         lst = np.ascontiguousarray(list(clouds), np.uint32)
         self._c(self.lib.j3dg_splat_points(self.ctx, int(target), ffi.ptr(P), ffi.ptr(ci), int(flags), ffi.ptr(lst),
                                            len(lst)), "j3dg_splat_points")

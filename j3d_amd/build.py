@@ -29,7 +29,7 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-flush-denormals-to
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-value"]
 
-HIP_SOURCES = ["j3dg.hip", "mlp.hip", "batch.hip", "bvh_gpu.hip", "allhits.hip", "splat.hip"]
+HIP_SOURCES = ["j3dg.hip", "mlp.hip", "bvh_gpu.hip", "allhits.hip", "splat.hip"]
 CPP_SOURCES = ["bvh_build.cpp", "scenegen.cpp", "host_tables.cpp", "autotune.cpp"]
 
 
@@ -54,12 +54,14 @@ def build_library(verbose=False):
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
-            continue
+            raise RuntimeError("build: listed source %s is missing" % path)
         obj = os.path.join(OBJDIR, src + ".o")
         jobs.append([HIPCC] + COMMON + HIP_FLAGS + ["-c", "-o", obj, path])
         objs.append(obj)
     for src in CPP_SOURCES:
         path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            raise RuntimeError("build: listed source %s is missing" % path)
         obj = os.path.join(OBJDIR, src + ".o")
         jobs.append([CXX] + COMMON + ["-pthread", "-c", "-o", obj, path])
         objs.append(obj)

@@ -961,6 +961,17 @@ int Ctx::ensure_spill(uint64_t lanes, uint32_t entries) {
 
 int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width) {
   std::vector<DevObject> tab(n);
+  int rc = fill_object_table(slots, n, tab.data(), max_stack, width);
+  if (rc) return rc;
+  if (n > J3DG_MAX_OBJECTS) {
+    rc = build_top_level(slots, n);
+    if (rc) return rc;
+    max_stack += top_depth;  // top-level entries sit below an object's
+  }
+  return upload_object_table(tab.data(), n);
+}
+
+int Ctx::fill_object_table(const uint32_t* slots, uint32_t n, DevObject* tab, uint32_t& max_stack, int& width) {
   max_stack = 0;
   width = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -989,12 +1000,7 @@ int Ctx::build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_sta
     std::memcpy(o.mlp_norm, m.mlp_norm, sizeof(o.mlp_norm));
     max_stack = std::max(max_stack, m.max_stack);
   }
-  if (n > J3DG_MAX_OBJECTS) {
-    int rc = build_top_level(slots, n);
-    if (rc) return rc;
-    max_stack += top_depth;  // top-level entries sit below an object's
-  }
-  return upload_object_table(tab.data(), n);
+  return J3DG_OK;
 }
 
 // World box of a local box under the column-major pose cs, in double, then
@@ -1453,6 +1459,7 @@ static int upload_mesh_device(Ctx* ctx, uint32_t db_id, const float* xyz, uint64
 extern "C" {
 
 int j3dg_abi_version(void) { return J3DG_ABI_VERSION; }
+
 
 int j3dg_device_count(int* count) {
   if (!count) return J3DG_ERR_INVALID_ARG;

@@ -263,6 +263,8 @@ struct Ctx {
   int fail(int code, const std::string& msg);
   int ensure_spill(uint64_t lanes, uint32_t entries);
   int build_object_table(const uint32_t* slots, uint32_t n, uint32_t& max_stack, int& width);
+  // the device records of the listed meshes into tab[0..n) (host memory; nothing allocated)
+  int fill_object_table(const uint32_t* slots, uint32_t n, DevObject* tab, uint32_t& max_stack, int& width);
   int build_top_level(const uint32_t* slots, uint32_t n);
   // scratch buffer i (0..3) of at least `bytes`, kept for later calls; nullptr when allocation fails
   void* scratch_buffer(int i, size_t bytes) {

@@ -19,6 +19,17 @@ void transform_point(const float4x4& m, const float p[3], float out[3]) {
   for (int r = 0; r < 3; ++r) out[r] = ((m[r] * p[0] + m[4 + r] * p[1]) + m[8 + r] * p[2]) + m[12 + r];
 }
 
+// compute_bb (j3d/mesh.cpp:39-58): std::min / std::max folded from the first point.
+void compute_bb(const std::vector<float>& xyz, float lo[3], float hi[3]) {
+  const size_t n = xyz.size() / 3;
+  for (int k = 0; k < 3; ++k) lo[k] = hi[k] = n ? xyz[k] : 0.f;
+  for (size_t i = 1; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], xyz[3 * i + k]);
+      hi[k] = std::max(hi[k], xyz[3 * i + k]);
+    }
+}
+
 matcap make_matcap(int type) {
   matcap m;
   m.im = image<uint32_t>(512, 512);
@@ -96,6 +107,7 @@ void add_pointcloud(uint32_t id, scene& s, const std::vector<float>& vertices, c
   pc.db_id = id;
   pc.cs = cs;
   pc.device_cs = cs;
+  compute_bb(vertices, pc.min_bb, pc.max_bb);  // j3d/scene.cpp:34
   s.dev->check(j3dg_upload_points(s.dev->ctx(), id, vertices.data(), n,
                                   normals && !normals->empty() ? normals->data() : nullptr,
                                   colors && !colors->empty() ? colors->data() : nullptr, cs.data(), &pc.cloud),
@@ -122,16 +134,8 @@ void add_object(uint32_t id, scene& s, const std::vector<float>& vertices, const
   obj.p_triangles = &triangles;
   obj.cs = cs;
   obj.device_cs = cs;
-  // compute_bb (j3d/mesh.cpp:39-58)
   const size_t nv = vertices.size() / 3;
-  if (nv) {
-    for (int k = 0; k < 3; ++k) obj.min_bb[k] = obj.max_bb[k] = vertices[k];
-    for (size_t i = 1; i < nv; ++i)
-      for (int k = 0; k < 3; ++k) {
-        obj.min_bb[k] = std::min(obj.min_bb[k], vertices[3 * i + k]);
-        obj.max_bb[k] = std::max(obj.max_bb[k], vertices[3 * i + k]);
-      }
-  }
+  compute_bb(vertices, obj.min_bb, obj.max_bb);
   j3dg_build_opts bo{bvh_width, 0, 0, builder};
   s.dev->check(j3dg_upload_mesh(s.dev->ctx(), id, vertices.data(), nv, triangles.data(), triangles.size() / 3,
                                 cs.data(), &bo, &obj.slot),
@@ -156,33 +160,39 @@ void set_object_colors(uint32_t id, scene& s, const std::vector<float>* vertex_c
                "j3dg_set_mesh_colors");
 }
 
-// j3d/scene.cpp:41-49
+// j3d/scene.cpp:41-49: the object and the point cloud with this id
 void remove_object(uint32_t id, scene& s) {
   auto it = std::find_if(s.objects.begin(), s.objects.end(), [&](const scene_object& so) { return so.db_id == id; });
   if (it != s.objects.end()) {
     s.dev->check(j3dg_remove_mesh(s.dev->ctx(), it->slot), "j3dg_remove_mesh");
     s.objects.erase(it);
   }
+  remove_pointcloud(id, s);
 }
 
-// j3d/scene.cpp:50-88
+// j3d/scene.cpp:50-88: the objects' and then the point clouds' world boxes,
+// starting from the first object's (or, without objects, the first cloud's)
 void prepare_scene(scene& s) {
-  auto box = [](const scene_object& o, float lo[3], float hi[3]) {
+  auto box = [](const float4x4& cs, const float* mn, const float* mx, float lo[3], float hi[3]) {
     float a[3], b[3];
-    transform_point(o.cs, o.min_bb, a);
-    transform_point(o.cs, o.max_bb, b);
+    transform_point(cs, mn, a);
+    transform_point(cs, mx, b);
     for (int k = 0; k < 3; ++k) lo[k] = std::min(a[k], b[k]), hi[k] = std::max(a[k], b[k]);
   };
+  auto grow = [&](const float4x4& cs, const float* mn, const float* mx) {
+    float lo[3], hi[3];
+    box(cs, mn, mx, lo, hi);
+    for (int k = 0; k < 3; ++k) s.min_bb[k] = std::min(s.min_bb[k], lo[k]), s.max_bb[k] = std::max(s.max_bb[k], hi[k]);
+  };
   if (!s.objects.empty()) {
-    box(s.objects.front(), s.min_bb, s.max_bb);
+    box(s.objects.front().cs, s.objects.front().min_bb, s.objects.front().max_bb, s.min_bb, s.max_bb);
+  } else if (!s.pointclouds.empty()) {
+    box(s.pointclouds.front().cs, s.pointclouds.front().min_bb, s.pointclouds.front().max_bb, s.min_bb, s.max_bb);
   } else {
     for (int k = 0; k < 3; ++k) s.min_bb[k] = s.max_bb[k] = 0.f;
   }
-  for (const auto& o : s.objects) {
-    float lo[3], hi[3];
-    box(o, lo, hi);
-    for (int k = 0; k < 3; ++k) s.min_bb[k] = std::min(s.min_bb[k], lo[k]), s.max_bb[k] = std::max(s.max_bb[k], hi[k]);
-  }
+  for (const auto& o : s.objects) grow(o.cs, o.min_bb, o.max_bb);
+  for (const auto& p : s.pointclouds) grow(p.cs, p.min_bb, p.max_bb);
   s.diagonal = s.max_bb[0] - s.min_bb[0];
   s.diagonal = std::max<float>(s.diagonal, s.max_bb[1] - s.min_bb[1]);
   s.diagonal = std::max<float>(s.diagonal, s.max_bb[2] - s.min_bb[2]);

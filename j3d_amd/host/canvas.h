@@ -132,6 +132,7 @@ struct scene_object {
 // (0xAABBGGRR), copied to the device by add_pointcloud.
 struct scene_pointcloud {
   uint32_t db_id = 0;
+  float min_bb[3] = {0, 0, 0}, max_bb[3] = {0, 0, 0};  // compute_bb of the points (j3d/scene.cpp:34)
   float4x4 cs = get_identity();
   uint32_t cloud = 0;  // device cloud id
   float4x4 device_cs = get_identity();
@@ -169,7 +170,7 @@ void set_object_colors(uint32_t id, scene& s, const std::vector<float>* vertex_c
 void add_pointcloud(uint32_t id, scene& s, const std::vector<float>& vertices, const std::vector<float>* normals,
                     const std::vector<uint32_t>* colors, const float4x4& cs = get_identity());
 void remove_pointcloud(uint32_t id, scene& s);
-// remove_object (j3d/scene.cpp:41-49)
+// remove_object (j3d/scene.cpp:41-49): the object and the point cloud with this id
 void remove_object(uint32_t id, scene& s);
 // prepare_scene / unzoom (j3d/scene.cpp:50-110)
 void prepare_scene(scene& s);

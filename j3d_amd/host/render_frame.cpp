@@ -247,9 +247,9 @@ int main(int argc, char** argv) {
     std::printf(
         "{\"width\": %u, \"height\": %u, \"triangles\": %zu, \"frames\": %u, \"upload_s\": %.3f, "
         "\"ms_per_frame_with_readback\": %.3f, \"trace_ms\": %.4f, \"shadow_ms\": %.4f, \"mlp_ms\": %.4f, "
-        "\"shade_ms\": %.4f}\n",
+        "\"shade_ms\": %.4f, \"diagonal\": %.9g, \"pivot\": [%.9g, %.9g, %.9g]}\n",
         w, h, tri.size() / 3, frames, std::chrono::duration<double>(t1 - t0).count(), wall / frames * 1e3, tr, sd, ml,
-        sh);
+        sh, sc.diagonal, sc.pivot[0], sc.pivot[1], sc.pivot[2]);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "j3d_render_frame: %s\n", e.what());
     return 1;

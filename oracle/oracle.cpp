@@ -684,7 +684,9 @@ void orc_bbox(const float* v, uint64_t nv, float* mn, float* mx) {
 
 void orc_prepare_and_unzoom(uint32_t nobj, const float* cs16, const float* bbmin3, const float* bbmax3,
                             float* diagonal, float* pivot, float* cs, float* cs_inv) {
-  /* j3d/scene.cpp:50-89 (objects only) and unzoom 91-110 (y axis up) */
+  /* j3d/scene.cpp:50-89 and unzoom 91-110 (y axis up).  The boxes are the scene's
+     objects followed by its point clouds: the first one is the reference's
+     starting box (the first object's, or without objects the first cloud's). */
   float smin[3] = {0, 0, 0}, smax[3] = {0, 0, 0};
   for (uint32_t i = 0; i < nobj; ++i) {
     const float* m = cs16 + 16 * i;

@@ -80,7 +80,8 @@ void orc_fill_background(uint32_t* im, uint32_t w, uint32_t h, uint32_t top, uin
 void orc_triangle_normals(const float* v, const uint32_t* t, uint64_t nt, float* n);
 void orc_bbox(const float* v, uint64_t nv, float mn[3], float mx[3]);
 
-/* Scene framing, j3d/scene.cpp:50-110: objects' (cs, bbox) -> diagonal, pivot, camera cs. */
+/* Scene framing, j3d/scene.cpp:50-110: the (cs, bbox) of the scene's objects, then of its
+   point clouds -> diagonal, pivot, camera cs. */
 void orc_prepare_and_unzoom(uint32_t nobj, const float* cs16, const float* bbmin3, const float* bbmax3,
                             float* diagonal, float pivot[3], float cs[16], float cs_inv[16]);
 

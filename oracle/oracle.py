@@ -158,6 +158,22 @@ def background(w, h, top=0xff000000, bottom=0xff404040):
     return im
 
 
+def prepare_and_unzoom(boxes):
+    """prepare_scene + unzoom (j3d/scene.cpp:50-110) over a list of (cs, bb_min,
+    bb_max): the scene's objects, then its point clouds.  Returns (diagonal,
+    pivot, cs, cs_inv)."""
+    n = len(boxes)
+    cs16 = np.ascontiguousarray(np.concatenate([np.asarray(b[0], np.float32).reshape(16) for b in boxes]), np.float32)
+    lo = np.ascontiguousarray(np.concatenate([np.asarray(b[1], np.float32).reshape(3) for b in boxes]), np.float32)
+    hi = np.ascontiguousarray(np.concatenate([np.asarray(b[2], np.float32).reshape(3) for b in boxes]), np.float32)
+    d = np.zeros(1, np.float32)
+    pivot = np.zeros(3, np.float32)
+    cs = np.zeros(16, np.float32)
+    ci = np.zeros(16, np.float32)
+    load().orc_prepare_and_unzoom(n, _p(cs16), _p(lo), _p(hi), _p(d), _p(pivot), _p(cs), _p(ci))
+    return float(d[0]), pivot, cs, ci
+
+
 def triangle_normals(v, t):
     v = np.ascontiguousarray(v, np.float32)
     t = np.ascontiguousarray(t, np.uint32)

@@ -70,3 +70,23 @@ def random_rays(n, seed, center, radius, t_near=1e-3):
     r[:, 6] = t_near
     r[:, 7] = np.finfo(np.float32).max
     return r
+
+
+def pick_crop(gbuf, side):
+    """The side x side window, on a half-overlapping grid of positions, whose
+    hit fraction is closest to 3/4: mostly surface, with some silhouette."""
+    h, w = gbuf.shape
+    side = min(side, w, h)
+    hit = (gbuf["object_id"] != 0xFFFFFFFF).astype(np.int64)
+    ii = np.zeros((h + 1, w + 1), np.int64)
+    ii[1:, 1:] = hit.cumsum(0).cumsum(1)
+    best = None
+    step = max(1, side // 2)
+    for y0 in range(0, h - side + 1, step):
+        for x0 in range(0, w - side + 1, step):
+            n = ii[y0 + side, x0 + side] - ii[y0, x0 + side] - ii[y0 + side, x0] + ii[y0, x0]
+            d = abs(n / float(side * side) - 0.75)
+            if best is None or d < best[0]:
+                best = (d, x0, y0)
+    _, x0, y0 = best
+    return (x0, y0, x0 + side - 1, y0 + side - 1)

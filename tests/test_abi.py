@@ -98,3 +98,31 @@ def test_frame_object_list():
     assert big.n_objects == 40 and big.object_list
     got = np.ctypeslib.as_array((C.c_uint32 * 40).from_address(big.object_list))
     assert list(got) == slots
+
+
+def test_point_cloud_arguments_are_checked_before_the_c_call():
+    """Mismatched per-point arrays and matrices that are not 16 floats raise
+    ValueError in the Python layer (the C side copies n * 12 bytes of normals
+    and reads 16 floats per matrix); no context or GPU is involved."""
+    import j3d_amd as J
+    r = J.Renderer.__new__(J.Renderer)  # the checks run before any library call
+    r.ctx = None
+    p = np.zeros((10, 3), np.float32)
+    with pytest.raises(ValueError):
+        r.upload_points(p, normals=np.zeros((9, 3), np.float32))
+    with pytest.raises(ValueError):
+        r.upload_points(p, colors=np.zeros(11, np.uint32))
+    with pytest.raises(ValueError):
+        r.upload_points(p, cs=np.eye(3, dtype=np.float32))
+    with pytest.raises(ValueError):
+        r.set_points_cs(0, np.zeros(12, np.float32))
+    with pytest.raises(ValueError):
+        r.splat_points(0, np.zeros(16, np.float32), np.zeros(15, np.float32), 0, [0])
+    with pytest.raises(ValueError):
+        r.splat_points(0, np.zeros(9, np.float32), np.zeros(16, np.float32), 0, [0])
+
+
+def test_build_lists_only_existing_sources():
+    from j3d_amd import build as B
+    for s in B.HIP_SOURCES + B.CPP_SOURCES:
+        assert os.path.exists(os.path.join(B.CSRC, s)), s

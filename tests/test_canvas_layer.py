@@ -144,3 +144,48 @@ def test_canvas_many_objects_matches_oracle(tmp_path, device_build):
     assert compare_rgba(im, oim)[0] <= 1
     hit = g["object_id"] != 0xFFFFFFFF
     assert len(np.unique(g["db_id"][hit])) > 16
+
+
+@pytest.mark.gpu
+def test_canvas_points_in_scene_framing(tmp_path):
+    """prepare_scene + unzoom through the canvas layer (no --cs) for a scene of
+    one mesh and one point cloud (j3d/scene.cpp:50-110): the cloud's box, which
+    the jitter pushes past the mesh's, enters the diagonal and the pivot, and so
+    the camera, t_near and the frame.  The frame and the points drawn over it
+    equal the oracle's with the framing it computes over both boxes."""
+    w, h = 256, 192
+    cam = J.camera_with_vertical_fov(60.0)
+    v, t = J.torus(300, 170, seed=2)
+    prefix = str(tmp_path / "pts")
+    cmd = [EXE, "--scene", "torus", "300", "170", "2", "--size", str(w), str(h), "--focal",
+           repr(float(cam.focal_length)), "--flags", "edges,shading", "--points", "60000", "5", "--out", prefix]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    import json
+    rep = json.loads(r.stdout.strip().splitlines()[-1])
+    rec = np.fromfile(prefix + ".points", np.float32).reshape(2, -1, 3)
+    col = np.fromfile(prefix + ".colors", np.uint32)
+    ident = np.eye(4, dtype=np.float32).reshape(16)
+    mlo, mhi = J.bbox(v)
+    plo, phi = J.bbox(rec[0])
+    diag, pivot, cs, ci = O.prepare_and_unzoom([(ident, mlo, mhi), (ident, plo, phi)])
+    diag_mesh, _, _, _ = O.prepare_and_unzoom([(ident, mlo, mhi)])
+    assert diag != diag_mesh  # the cloud changes the framing
+    assert np.float32(rep["diagonal"]) == np.float32(diag)
+    assert np.array_equal(np.array(rep["pivot"], np.float32), pivot)
+    flags = ffi.EDGES | ffi.SHADING
+    fr = J.Frame(w, h, J.projection(cam, w, h)[1], cs, ci, pivot, diag, [0], flags=flags,
+                 near_plane=cam.near_plane)
+    g = np.fromfile(prefix + ".gbuf", ffi.PIXEL_DTYPE).reshape(h, w)
+    im = np.fromfile(prefix + ".rgba", np.uint32).reshape(h, w)
+    mc, cav = O.matcap(0)
+    og = np.zeros((h, w), O.PIXEL_DTYPE)
+    og["object_id"] = 0xFFFFFFFF
+    og, _ = O.update_canvas([O.Mesh(v, t, db_id=1 << 29)], fr, gbuf=og)
+    oim = O.canvas_to_image(og, fr, mc, cav)
+    P, _ = J.projection(cam, w, h)
+    og, oim = O.splat_points(og, oim, P, ci, flags, [{"xyz": rec[0], "normals": rec[1], "colors": col,
+                                                      "db_id": (2 << 29) + 1}])
+    assert (g["db_id"] == (2 << 29) + 1).sum() > 500
+    assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
+    assert compare_rgba(im, oim)[0] <= 1
