@@ -1,0 +1,64 @@
+"""The batch path (j3dg_render_batch: many frames in one persistent k_trace
+launch + k_shade_batch, captured in a graph) run on the CPU before the GPU.
+
+j3d_amd/csrc/j3dg.hip is compiled for x86 against a stand-in HIP runtime
+(tests/emu: fibers in lockstep per 64-lane wave, every device allocation
+fenced by unmapped memory, a fault handler naming the allocation an access
+fell outside of) and driven through the same C ABI by tests/emu/batch_case.py:
+13 frames of 5 meshes, five sizes, sub-rectangles and every shading mode,
+rendered alone and as a batch (capture, replay, updated frame tables), each
+batch frame equal to its single render and to the oracle.
+
+The two mistakes that faulted the GPU with an illegal memory access are
+planted in copies of the source and must fault here too, while the current
+source runs clean:
+  wrong_table       the batch launch passed the single-frame object table;
+  no_frame_handoff  the cooperative tail moved a ray's state but not its frame
+                    (index, object table, object count), so a lane completed
+                    another frame's hit with the wrong object's records.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "emu")
+sys.path.insert(0, EMU)
+import build_emu  # noqa: E402
+
+pytestmark = pytest.mark.skipif(build_emu.compiler() is None, reason="needs clang++")
+
+VARIANTS = [None, "wrong_table", "no_frame_handoff"]
+
+
+@pytest.fixture(scope="module")
+def libs(tmp_path_factory):
+    base = tmp_path_factory.mktemp("emu")
+
+    def one(v):
+        return v, build_emu.build(str(base / (v or "current")), v)
+    with ThreadPoolExecutor(3) as ex:
+        return dict(ex.map(one, VARIANTS))
+
+
+def _run(lib):
+    env = dict(os.environ, J3DG_LIBRARY=lib)
+    return subprocess.run([sys.executable, os.path.join(EMU, "batch_case.py")], env=env, stdout=subprocess.PIPE,
+                          stderr=subprocess.STDOUT, text=True, timeout=600)
+
+
+def test_batch_on_the_cpu_equals_single_frames_and_oracle(libs):
+    r = _run(libs[None])
+    print(r.stdout[-3000:])
+    assert r.returncode == 0 and "BATCH OK" in r.stdout, r.stdout[-3000:]
+    assert "shuffles that read a lane outside the exec mask: 0" in r.stdout
+
+
+@pytest.mark.parametrize("variant", ["wrong_table", "no_frame_handoff"])
+def test_planted_batch_mistakes_fault_on_the_cpu(libs, variant):
+    r = _run(libs[variant])
+    print(r.stdout[-3000:])
+    assert r.returncode == 86 and "EMU FAULT" in r.stdout and "in kernel k_trace" in r.stdout, r.stdout[-3000:]
