@@ -256,17 +256,29 @@ int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32
    Times the trace stages of `frame` (median of `repeats` frames, default 5)
    over LDS stack depth {8,16,32} x rays per work item {1,2,4}, applies the
    fastest tuning to the context and, for a J3DG_LEARNED frame, also picks the
-   fastest bound margin (returned; the caller puts it in its frames).  BVH
-   width {4,8} is the caller's choice between two uploads of the mesh. */
+   fastest bound margin (returned; the caller puts it in its frames).  The
+   report's width is that of the frame's meshes. */
 typedef struct j3dg_autotune_report {
   j3dg_tuning best;
   float best_margin;
-  float best_ms;      /* trace stages of the frame with the chosen settings */
+  float best_ms;      /* trace stages of the frame (or one batch launch) with the chosen settings */
   float baseline_ms;  /* with the tuning in effect before the call */
   uint32_t n_tried;
+  uint32_t width;     /* BVH width of the chosen frames (4 or 8) */
 } j3dg_autotune_report;
 int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, uint32_t repeats,
                   j3dg_autotune_report* report);
+/* BVH width {4,8} as well: the same frame twice, its objects uploaded once per
+   width (frame_w4 over width-4 slots, frame_w8 over width-8 slots; either may be
+   NULL).  Each is tuned as j3dg_autotune; the faster is applied and reported. */
+int j3dg_autotune_widths(j3dg_ctx* ctx, const j3dg_frame_desc* frame_w4, const j3dg_frame_desc* frame_w8,
+                         uint32_t target, uint32_t repeats, j3dg_autotune_report* report);
+/* A batch (j3dg_render_batch) over BVH width {4,8} x LDS stack depth {8,16,32} x
+   rays per work item {1,2,4}: the n frames given once per width (either list
+   may be NULL), timed as one graph launch (median of `repeats`, default 3);
+   applies the fastest tuning and reports its width. */
+int j3dg_autotune_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames_w4, const j3dg_frame_desc* frames_w8,
+                        const uint32_t* targets, uint32_t n, uint32_t repeats, j3dg_autotune_report* report);
 
 /* ---- learned proposal (no counterpart in the reference) ---------------------
    Trains the mesh's proposal network on the GPU: random-camera rays around the

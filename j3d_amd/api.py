@@ -444,8 +444,7 @@ class Renderer:
         rep = ffi.AutotuneReport()
         d = frame.desc()
         self._c(self.lib.j3dg_autotune(self.ctx, C.byref(d), int(target), int(repeats), C.byref(rep)), "j3dg_autotune")
-        out = {"tuning": {k: getattr(rep.best, k) for k, _ in ffi.Tuning._fields_}, "margin": rep.best_margin,
-               "best_ms": rep.best_ms, "baseline_ms": rep.baseline_ms, "n_tried": rep.n_tried, "cached": False}
+        out = self._report(rep)
         if cache and key:
             db = {}
             if os.path.exists(cache):
@@ -455,6 +454,35 @@ class Renderer:
             with open(cache, "w") as f:
                 json.dump(db, f, indent=1, sort_keys=True)
         return out
+
+    @staticmethod
+    def _report(rep):
+        return {"tuning": {k: getattr(rep.best, k) for k, _ in ffi.Tuning._fields_}, "margin": rep.best_margin,
+                "best_ms": rep.best_ms, "baseline_ms": rep.baseline_ms, "n_tried": rep.n_tried,
+                "width": rep.width, "cached": False}
+
+    def autotune_widths(self, frame_w4, frame_w8, target, repeats=5):
+        """j3dg_autotune_widths: the same frame over width-4 and width-8 uploads
+        of its meshes (either may be None); the faster tuning is applied."""
+        rep = ffi.AutotuneReport()
+        d4 = frame_w4.desc() if frame_w4 is not None else None
+        d8 = frame_w8.desc() if frame_w8 is not None else None
+        self._c(self.lib.j3dg_autotune_widths(self.ctx, C.byref(d4) if d4 is not None else None,
+                                              C.byref(d8) if d8 is not None else None, int(target), int(repeats),
+                                              C.byref(rep)), "j3dg_autotune_widths")
+        return self._report(rep)
+
+    def autotune_batch(self, frames_w4, frames_w8, targets, repeats=3):
+        """j3dg_autotune_batch: BVH width x stack depth x rays per work item
+        on one graph launch of the batch; applies the fastest tuning."""
+        n = len(targets)
+        a4 = (ffi.FrameDesc * n)(*[f.desc() for f in frames_w4]) if frames_w4 is not None else None
+        a8 = (ffi.FrameDesc * n)(*[f.desc() for f in frames_w8]) if frames_w8 is not None else None
+        tg = np.ascontiguousarray(targets, np.uint32)
+        rep = ffi.AutotuneReport()
+        self._c(self.lib.j3dg_autotune_batch(self.ctx, a4, a8, ffi.ptr(tg), n, int(repeats), C.byref(rep)),
+                "j3dg_autotune_batch")
+        return self._report(rep)
 
     # ---- learned proposal ------------------------------------------------
     def train_proposal(self, slot, **opts):

@@ -2187,14 +2187,10 @@ int j3dg_debug_timeline(j3dg_ctx* h, uint64_t* out, uint32_t max_waves, uint32_t
 // (kModeBatch) takes tiles of all frames from one queue, one k_shade_batch
 // shades them; reset + trace + shade are captured in a HIP graph and replayed
 // while the launch parameters are unchanged (the tables' contents may change).
-// Switched off: the batch launch faulted the GPU (illegal memory access) on
-// its first two runs.  Causes found by reading the code since: the launch
-// passed the single-frame object table instead of the batch's, and the
-// cooperative tail hand-off did not move a ray's frame (index, object table,
-// object count) to the lanes that take it over, so a writer lane could store
-// into another frame's G-buffer out of bounds.  Both are fixed below; the path
-// stays off until a GPU run has verified the fixes.
-static constexpr bool kBatchEnabled = false;
+// The batch launch reads the batch's object table (d_bobjs), and the
+// cooperative tail hands a ray's frame (index, object table, object count) to
+// the lanes that take it over together with its state.
+static constexpr bool kBatchEnabled = true;
 
 int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n) {
   Ctx* ctx = reinterpret_cast<Ctx*>(h);

@@ -88,7 +88,7 @@ class Stats(C.Structure):
 
 class AutotuneReport(C.Structure):
     _fields_ = [("best", Tuning), ("best_margin", C.c_float), ("best_ms", C.c_float), ("baseline_ms", C.c_float),
-                ("n_tried", C.c_uint32)]
+                ("n_tried", C.c_uint32), ("width", C.c_uint32)]
 
 
 class TrainOpts(C.Structure):
@@ -158,6 +158,9 @@ SIGNATURES = {
     "j3dg_calibration_copy": (_I, [_P, _U64, _U32, C.POINTER(_F)]),
     "j3dg_debug_timeline": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
     "j3dg_autotune": (_I, [_P, C.POINTER(FrameDesc), _U32, _U32, C.POINTER(AutotuneReport)]),
+    "j3dg_autotune_widths": (_I, [_P, C.POINTER(FrameDesc), C.POINTER(FrameDesc), _U32, _U32,
+                                  C.POINTER(AutotuneReport)]),
+    "j3dg_autotune_batch": (_I, [_P, _P, _P, _P, _U32, _U32, C.POINTER(AutotuneReport)]),
     "j3dg_default_train_opts": (_I, [C.POINTER(TrainOpts)]),
     "j3dg_train_proposal": (_I, [_P, _U32, C.POINTER(TrainOpts), C.POINTER(TrainReport)]),
     "j3dg_get_proposal": (_I, [_P, _U32, _P, _P]),
