@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--train-rays", type=int, default=None, help="default 2^24 (2^25 for torus300m)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--autotune", type=int, default=1, help="autotune stack depth / rays per item / margin")
+    ap.add_argument("--orbit-frames", type=int, default=24,
+                    help="torus28m: also time an orbiting camera of this many frames per turn (0: off)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with a 1 GiB device copy (read + write), the reference for profiler byte counters")
     return ap.parse_args()
@@ -137,8 +139,17 @@ def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, min_seconds):
     m = O.Mesh(v, t, db_id=(1 << 29), threads=threads)
     bvh_s = time.time() - t0
     h, w = frame.height, frame.width
+    # parity frame (not timed): every G-buffer field against the GPU's, near
+    # ties (the two nearest hits within 1e-6 relative depth) counted
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scenes import compare_gbuffers
+    g = np.zeros((h, w), O.PIXEL_DTYPE)
+    g["object_id"] = 0xFFFFFFFF
+    g, tst = O.update_canvas([m], frame, gbuf=g, threads=threads, count_ties=True)
+    im = O.canvas_to_image(g, frame, mc, cav)
+    field_mism = {k: n for k, n in compare_gbuffers(gpu_gbuf, g).items() if n}
+    near_ties = int(tst["near_ties"])
     frames, rays, dt = 0, 0, 0.0
-    g = None
     while dt < min_seconds or frames == 0:
         g = np.zeros((h, w), O.PIXEL_DTYPE)
         g["object_id"] = 0xFFFFFFFF
@@ -150,9 +161,7 @@ def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, min_seconds):
         frames += 1
         if frames >= 500:
             break
-    mism = int((g["object_id"] != gpu_gbuf["object_id"]).sum() +
-               (g["depth"].view(np.uint32) != gpu_gbuf["depth"].view(np.uint32)).sum() +
-               (g["mark"] != gpu_gbuf["mark"]).sum())
+    mism = int(sum(field_mism.values()))
     d = np.abs(im.view(np.uint8).astype(int) - gpu_im.view(np.uint8).astype(int))
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -162,8 +171,43 @@ def cpu_baseline(v, t, frame, mc, cav, gpu_gbuf, gpu_im, min_seconds):
             "sample": "%d full %dx%d frames (%d primary + %d shadow rays each), update_canvas + canvas_to_image, "
                       "%.1fs of CPU time (oracle BVH build %.1fs excluded)" % (frames, w, h, st["primary_rays"],
                                                                              st["shadow_rays"], dt, bvh_s),
-            "cpu_model": model, "gbuffer_mismatches_vs_gpu": mism, "rgba_max_diff_vs_gpu": int(d.max()),
+            "cpu_model": model, "gbuffer_mismatches_vs_gpu": mism, "gbuffer_field_mismatches": field_mism,
+            "near_tie_pixels": near_ties, "rgba_max_diff_vs_gpu": int(d.max()),
             "ms_per_frame": dt / frames * 1e3}
+
+
+def orbit_sequence(r, tg, cam, pivot, diag, slot, flags, light, w, h, n, passes):
+    """A viewer's orbiting camera: n frames whose camera moves 360/n degrees in
+    azimuth from one to the next (distance 3, elevation 35 degrees), each
+    frame's tile cost order predicted from the frame before it (another
+    view).  Counts the rays of every frame, then times `passes` passes over
+    the sequence (plain traversal)."""
+    import j3d_amd as J
+    frames = []
+    for k in range(n):
+        cs, ci = J.orbit_camera(3.0, 2.0 * math.pi * k / n, math.radians(35.0))
+        frames.append(J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light))
+    r.set_counting(True)
+    rays = 0
+    for f in frames:
+        r.render(f, tg, readback=False)
+        st = r.stats()
+        rays += st["primary_rays"] + st["shadow_rays"]
+    r.set_counting(False)
+    for f in frames:  # warm-up pass
+        r.render(f, tg, readback=False)
+    r.synchronize()
+    r.reset_timings()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for f in frames:
+            r.render(f, tg, readback=False)
+    r.synchronize()
+    wall = time.perf_counter() - t0
+    tm = r.timings()
+    return {"frames": n, "degrees_per_frame": 360.0 / n, "passes": passes, "rays_per_pass": int(rays),
+            "mrays": rays * passes / wall / 1e6, "ms_per_frame": wall / (passes * n) * 1e3,
+            "trace_ms_mean": float(np.mean(tm["trace_ms"]))}
 
 
 def orbit_cameras(n, seed):
@@ -412,11 +456,14 @@ def mesh_set_bench(a):
     """16 meshes of uneven size (displaced tori, 1e4..1e7 triangles, log-uniform)
     and 64 orbit cameras at resolutions drawn from {256x256, 640x480, 1280x720,
     1920x1080}; camera i shows mesh i mod 16 (single-object frames), edges +
-    matcap.  Frames render one after another (multi-frame batching is switched
-    off).  A sweep over BVH width {4, 8} x rays per work item {1, 2, 4} x LDS
-    stack depth {8, 16, 32} times all 64 frames per configuration and keeps the
-    fastest.  Parity: the width-4 and width-8 G-buffers of every frame must be
-    identical, and a 64 x 64 crop of every frame must match the oracle."""
+    matcap.  A step renders all 64 frames in one persistent k_trace launch +
+    one k_shade_batch, captured in a HIP graph and replayed
+    (j3dg_render_batch).  The autotuner (j3dg_autotune_batch) sweeps BVH width
+    {4, 8} x rays per work item {1, 2, 4} x LDS stack depth {8, 16, 32} on that
+    one launch.  Parity: every batch frame equal to the same frame rendered
+    alone, the width-4 and width-8 frames identical, and a 64 x 64 crop of every
+    frame against the oracle.  The frames rendered one after another (64
+    launches) are timed beside it."""
     import ctypes as C
     import j3d_amd as J
     from j3d_amd import ffi
@@ -457,38 +504,22 @@ def mesh_set_bench(a):
         for w in (4, 8):
             frames[w].append(J.make_frame(fw, fh, cam, cs, ci, pivot, diag, [slots[w][i % 16]], flags=flags))
     targets = [r.create_target(fw, fh) for _, _, (fw, fh) in cams]
+    npx = sum(fw * fh for _, _, (fw, fh) in cams)
 
-    def pass_ms(frs):
-        r.synchronize()
-        t1 = time.perf_counter()
-        for f, tg in zip(frs, targets):
-            r.render(f, tg, readback=False)
-        r.synchronize()
-        return (time.perf_counter() - t1) * 1e3
+    tune = r.autotune_batch(frames[4], frames[8], targets, repeats=3)
+    log("batch autotune: %s" % tune)
+    bw = tune["width"]
+    bf = frames[bw]
 
-    base = r.get_tuning()
-    sweep = []
-    for w in (4, 8):
-        for depth in (8, 16, 32):
-            for rpi in (1, 2, 4):
-                r.set_tuning(**dict(base, stack_depth=depth, rays_per_item=rpi))
-                pass_ms(frames[w])
-                ms = min(pass_ms(frames[w]) for _ in range(2))
-                sweep.append({"width": w, "stack_depth": depth, "rays_per_item": rpi, "ms": ms})
-    best = min(sweep, key=lambda e: e["ms"])
-    log("sweep best: %s (worst %.2f ms)" % (best, max(e["ms"] for e in sweep)))
-    r.set_tuning(**dict(base, stack_depth=best["stack_depth"], rays_per_item=best["rays_per_item"]))
-    bf = frames[best["width"]]
-
-    # rays per pass, and the G-buffers of both widths (must agree bit for bit)
+    # the frames one at a time: reference output and rays; then the batch
     r.set_counting(True)
     rays, hits, nodes, tris = 0, 0, 0, 0
-    outs = {4: [], 8: []}
+    single = {4: [], 8: []}
     for w in (4, 8):
         for f, tg in zip(frames[w], targets):
             g, im = r.render(f, tg)
-            outs[w].append((g, im))
-            if w == best["width"]:
+            single[w].append((g, im))
+            if w == bw:
                 st = r.stats()
                 rays += st["primary_rays"] + st["shadow_rays"]
                 hits += st["hits"]
@@ -496,41 +527,62 @@ def mesh_set_bench(a):
                 tris += st["tri_tests"]
     r.set_counting(False)
     width_mism = 0
-    for (g4, i4), (g8, i8) in zip(outs[4], outs[8]):
-        width_mism += int(sum(np.count_nonzero(g4[k].view(np.uint32) != g8[k].view(np.uint32))
-                              if g4[k].dtype.itemsize == 4 else np.count_nonzero(g4[k] != g8[k])
-                              for k in g4.dtype.names)) + int(np.count_nonzero(i4 != i8))
+    for (g4, i4), (g8, i8) in zip(single[4], single[8]):
+        width_mism += sum(compare_gbuffers(g4, g8).values()) + int(np.count_nonzero(i4 != i8))
+    # every target holds its frame already: a batch over them must leave them the same
+    r.render_batch(bf, targets)
+    r.synchronize()
+    batch_mism = 0
+    outs = []
+    for (g0, im0), tg in zip(single[bw], targets):
+        g, im = r.read_target(tg)
+        outs.append((g, im))
+        batch_mism += sum(compare_gbuffers(g, g0).values()) + int(np.count_nonzero(im != im0))
+    log("batch vs single frames: %d mismatches; width 4 vs 8: %d" % (batch_mism, width_mism))
 
     for _ in range(a.warmup):
-        pass_ms(bf)
+        r.render_batch(bf, targets)
+    r.synchronize()
     r.reset_timings()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render_batch(bf, targets)
+    r.synchronize()
+    wall = time.perf_counter() - t1
+    tm = r.timings()
+    value = rays * a.steps / wall / 1e6
+    graph_ms = float(np.mean(tm["trace_ms"]))  # one graph launch: k_reset + k_trace + k_shade_batch
+    log("batch: %.1f Mrays/s, %.2f ms per launch of 64 frames (graph %.2f ms)" % (value, wall / a.steps * 1e3,
+                                                                               graph_ms))
+    # the same frames as 64 launches, for comparison
+    for f, tg in zip(bf, targets):
+        r.render(f, tg, readback=False)
+    r.synchronize()
     t1 = time.perf_counter()
     for _ in range(a.steps):
         for f, tg in zip(bf, targets):
             r.render(f, tg, readback=False)
     r.synchronize()
-    wall = time.perf_counter() - t1
-    tm = r.timings()
-    value = rays * a.steps / wall / 1e6
-    log("set: %.1f Mrays/s, %.2f ms per pass of 64 frames" % (value, wall / a.steps * 1e3))
-    node_bytes = 96 if best["width"] == 8 else 64
-    algo = nodes * node_bytes + tris * 48 + 32 * sum(fw * fh for _, _, (fw, fh) in cams)
-    trace_total_ms = float(np.sum(tm["trace_ms"])) / a.steps
-    achieved = algo / (trace_total_ms * 1e-3)
+    seq_wall = time.perf_counter() - t1
+    seq_value = rays * a.steps / seq_wall / 1e6
+    log("sequential: %.1f Mrays/s, %.2f ms per 64 frames" % (seq_value, seq_wall / a.steps * 1e3))
+    node_bytes = 96 if bw == 8 else 64
+    algo = nodes * node_bytes + tris * 48 + 32 * npx
+    achieved = algo / (graph_ms * 1e-3)
 
-    # oracle parity on a 64 x 64 crop of every frame
+    # oracle parity on a 64 x 64 crop of every batch frame
     import oracle as O
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     t0 = time.time()
     om = [O.Mesh(v, t, db_id=J.make_db_id(1, i), threads=threads) for i, (_, _, v, t) in enumerate(meshes)]
     bvh_s = time.time() - t0
     crop_mism, cpu_s, cpu_rays = {}, 0.0, 0
-    for i, (f, (g, im)) in enumerate(zip(bf, outs[best["width"]])):
+    for i, (f, (g, im)) in enumerate(zip(bf, outs)):
         fw, fh = f.width, f.height
         x0, y0 = fw // 2 - 32, fh // 2 - 32
         rect = (x0, y0, x0 + 63, y0 + 63)
         cf = J.make_frame(fw, fh, cam, cams[i][0], cams[i][1], framing[i % 16][1], framing[i % 16][0],
-                          [slots[best["width"]][i % 16]], flags=flags, rect=rect)
+                          [slots[bw][i % 16]], flags=flags, rect=rect)
         og = np.zeros((fh, fw), O.PIXEL_DTYPE)
         og["object_id"] = 0xFFFFFFFF
         t1 = time.time()
@@ -545,19 +597,20 @@ def mesh_set_bench(a):
                         "matcap" % (sum(n_tris), min(n_tris), max(n_tris)),
             "meshes": [[int(nu), int(nv), int(len(t))] for nu, nv, _, t in meshes],
             "resolutions": [list(c[2]) for c in cams],
-            "launches": "one persistent k_trace + k_shade per frame (multi-frame batching is switched off)"}
-    out = {"metric": "Mrays/s, 16 meshes x 64 cameras of mixed size, sequential frames", "value": value,
+            "launches": "all 64 frames in one persistent k_trace + one k_shade_batch, captured in a HIP graph"}
+    out = {"metric": "Mrays/s, 16 meshes x 64 cameras of mixed size in one graph-captured launch", "value": value,
            "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic", "config": desc,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK, "traffic": None,
-                        "kernel": "k_trace (64 launches per step, summed)", "launch_ms": trace_total_ms,
-                        "algorithmic_bytes_per_launch": int(algo),
+                        "kernel": "k_trace + k_shade_batch (one graph launch; the shading pass's time included)",
+                        "launch_ms": graph_ms, "algorithmic_bytes_per_launch": int(algo),
                         "per_unit": "%d B per node visit, 48 B per triangle test, 32 B G-buffer write per pixel"
                                     % node_bytes},
-           "rays_per_step": int(rays), "hit_pixels": int(hits), "sweep": sweep, "best": best,
-           "width4_vs_width8_mismatches": width_mism,
+           "rays_per_step": int(rays), "hit_pixels": int(hits), "autotune": tune, "width": bw,
+           "sequential_mrays": seq_value, "sequential_ms_per_step": seq_wall / a.steps * 1e3,
+           "batch_vs_single_mismatches": batch_mism, "width4_vs_width8_mismatches": width_mism,
            "parity": {"crop": "64x64 at every frame's centre", "gbuffer_mismatches": crop_mism},
            "cpu_baseline": {"value": cpu_rays / cpu_s / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
                             "sample": "64 crops of 64x64 pixels (%d rays), oracle update_canvas, %.2fs (oracle BVH "
@@ -874,6 +927,11 @@ def main():
     plain_value = rays_per_frame * a.steps / wall / 1e6
     ms_per_step, value = plain_ms, plain_value
 
+    orbit = None
+    if a.workload == "torus28m" and a.orbit_frames > 0:
+        orbit = orbit_sequence(r, tg, cam, pivot, diag, slot, flags, light, w, h, a.orbit_frames, a.steps)
+        log("orbit: %s" % orbit)
+
     learned = None
     if not a.no_learned:
         n_train = a.train_rays or (1 << 24)
@@ -932,6 +990,7 @@ def main():
            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": desc,
            "roofline": roof,
            "plain_mrays": plain_value, "plain_ms_per_step": plain_ms,
+           "orbit": orbit,
            "learned": learned,
            "learned_over_plain": (learned["mrays"] / plain_value) if learned else None,
            "autotune_plain": tune,

@@ -247,6 +247,13 @@ int j3dg_render(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, ui
                 j3dg_pixel* gbuf_out, uint32_t* rgba_out);
 /* Copy a target back: G-buffer and/or RGBA (either may be NULL). Synchronous. */
 int j3dg_read_target(j3dg_ctx* ctx, uint32_t target, j3dg_pixel* gbuf, uint32_t* rgba);
+/* As j3dg_read_target into images whose rows are `*_stride` elements apart
+   (jtk::image<pixel> / jtk::image<uint32_t> rows padded to their stride(), as
+   canvas.cpp:964-966 reads them): row y of the target goes to
+   gbuf + y * gbuf_stride and rgba + y * rgba_stride; strides >= the width.
+   The padding between rows is left untouched. */
+int j3dg_read_target_strided(j3dg_ctx* ctx, uint32_t target, j3dg_pixel* gbuf, uint32_t gbuf_stride,
+                             uint32_t* rgba, uint32_t rgba_stride);
 
 /* Many frames (cameras x meshes of uneven size) in one persistent launch,
    captured in a HIP graph and replayed while the batch is unchanged. */

@@ -358,6 +358,16 @@ class Renderer:
         self._c(self.lib.j3dg_read_target(self.ctx, int(t), ffi.ptr(g), ffi.ptr(im)), "j3dg_read_target")
         return g, im
 
+    def read_target_strided(self, t, gbuf_stride, rgba_stride):
+        """j3dg_read_target_strided into images of h rows of the given strides
+        (in elements); returns the padded arrays (padding left as zeros)."""
+        w, h = self._targets[int(t)]
+        g = np.zeros((h, gbuf_stride), ffi.PIXEL_DTYPE)
+        im = np.zeros((h, rgba_stride), np.uint32)
+        self._c(self.lib.j3dg_read_target_strided(self.ctx, int(t), ffi.ptr(g), int(gbuf_stride), ffi.ptr(im),
+                                                  int(rgba_stride)), "j3dg_read_target_strided")
+        return g, im
+
     def render_batch(self, frames, targets):
         arr = (ffi.FrameDesc * len(frames))(*[f.desc() for f in frames])
         tg = np.ascontiguousarray(targets, np.uint32)

@@ -1931,6 +1931,27 @@ int j3dg_read_target(j3dg_ctx* h, uint32_t target, j3dg_pixel* gbuf, uint32_t* r
   return J3DG_OK;
 }
 
+int j3dg_read_target_strided(j3dg_ctx* h, uint32_t target, j3dg_pixel* gbuf, uint32_t gbuf_stride, uint32_t* rgba,
+                             uint32_t rgba_stride) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (target >= ctx->targets.size() || !ctx->targets[target].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target");
+  Target& t = ctx->targets[target];
+  if ((gbuf && gbuf_stride < t.w) || (rgba && rgba_stride < t.w))
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "row stride below the target width");
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  // one 2-D copy per buffer: rows of w elements from the packed device image into the strided host one
+  if (gbuf)
+    HIP_CALL(hipMemcpy2D(gbuf, (size_t)gbuf_stride * sizeof(j3dg_pixel), t.gbuf, (size_t)t.w * sizeof(j3dg_pixel),
+                         (size_t)t.w * sizeof(j3dg_pixel), t.h, hipMemcpyDeviceToHost),
+             J3DG_ERR_LAUNCH);
+  if (rgba)
+    HIP_CALL(hipMemcpy2D(rgba, (size_t)rgba_stride * sizeof(uint32_t), t.rgba, (size_t)t.w * sizeof(uint32_t),
+                         (size_t)t.w * sizeof(uint32_t), t.h, hipMemcpyDeviceToHost),
+             J3DG_ERR_LAUNCH);
+  return J3DG_OK;
+}
+
 static int ensure_proposal(Ctx* ctx, Target& tg) {
   if (tg.proposal) return J3DG_OK;
   HIP_CALL(hipMalloc(&tg.proposal, (size_t)tg.w * tg.h * sizeof(float2)), J3DG_ERR_OUT_OF_MEMORY);

@@ -152,6 +152,7 @@ SIGNATURES = {
     "j3dg_target_device_ptrs": (_I, [_P, _U32, C.POINTER(_P), C.POINTER(_P)]),
     "j3dg_render": (_I, [_P, C.POINTER(FrameDesc), _U32, _U32, _P, _P]),
     "j3dg_read_target": (_I, [_P, _U32, _P, _P]),
+    "j3dg_read_target_strided": (_I, [_P, _U32, _P, _U32, _P, _U32]),
     "j3dg_render_batch": (_I, [_P, C.POINTER(FrameDesc), _P, _U32]),
     "j3dg_trace_rays": (_I, [_P, _P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     "j3dg_any_hit_rays": (_I, [_P, _P, _U32, _P, _U64, _P]),

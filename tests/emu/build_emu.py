@@ -81,7 +81,7 @@ def build(out, variant=None, jobs=4):
              "-fno-fast-math", "-w", "-I" + os.path.join(EMU), "-I" + src, "-I" + os.path.join(ROOT, "include"),
              "-I" + CSRC, "-pthread"]
     units = [os.path.join(src, "j3dg_emu.cpp"), os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_stubs.cpp")]
-    units += [os.path.join(CSRC, n) for n in ("bvh_build.cpp", "scenegen.cpp", "host_tables.cpp")]
+    units += [os.path.join(CSRC, n) for n in ("bvh_build.cpp", "scenegen.cpp", "host_tables.cpp", "autotune.cpp")]
     objs, procs = [], []
     for u in units:
         o = os.path.join(out, os.path.basename(u) + ".o")

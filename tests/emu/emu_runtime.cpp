@@ -2,6 +2,7 @@
 // fibers in lockstep per wave for the kernels, guard-slot allocations for
 // device memory, a fault handler that reports the access, stream capture.
 // Test infrastructure only.
+#include <execinfo.h>
 #include <signal.h>
 #include <sys/mman.h>
 #include <ucontext.h>
@@ -144,16 +145,18 @@ void fault_handler(int sig, siginfo_t* si, void*) {
   long long off = 0;
   uint64_t serial = 0;
   size_t bytes = 0;
+  bool freed = false;
   if (g_arena && a >= g_arena && a < g_arena + kSlot * kSlots) {
     const size_t s = (size_t)(a - g_arena) / kSlot;
-    // the allocation this slot holds, or the one before it (an access past that one's end)
+    // the allocation this slot holds (or held: a freed one), or the one before it (an access past its end)
     for (long long j = (long long)s; j >= 0 && j >= (long long)s - 1; --j) {
       const Alloc& al = allocs()[(size_t)j];
-      if (al.live) {
+      if (al.start) {
         k = (int)j;
         off = (long long)(a - al.start);
         serial = al.serial;
         bytes = al.bytes;
+        freed = !al.live;
         break;
       }
     }
@@ -161,9 +164,13 @@ void fault_handler(int sig, siginfo_t* si, void*) {
   int n = snprintf(msg, sizeof(msg),
                    "EMU FAULT: signal %d at %p in kernel %s, block (%u,%u,%u) thread %u: %s allocation #%llu "
                    "(%zu bytes) at offset %lld\n",
-                   sig, (const void*)a, g_kernel, g_cur_block.x, g_cur_block.y, g_cur_block.z,
-                   g_cur ? g_cur->tid.x : 0u, k >= 0 ? "outside" : "no", (unsigned long long)serial, bytes, off);
+                   sig, (const void*)a, g_kernel, g_cur ? g_cur->block.x : 0u, g_cur ? g_cur->block.y : 0u,
+                   g_cur ? g_cur->block.z : 0u, g_cur ? g_cur->tid.x : 0u,
+                   k < 0 ? "no" : freed ? "a freed" : "outside", (unsigned long long)serial, bytes, off);
   if (n > 0) (void)!write(2, msg, (size_t)std::min(n, (int)sizeof(msg) - 1));
+  void* frames[48];
+  const int depth = backtrace(frames, 48);
+  backtrace_symbols_fd(frames, depth, 2);
   _exit(86);
 }
 
@@ -377,7 +384,14 @@ hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipMalloc(void** p, size_t bytes) {
   char* base = arena();
   if (bytes > kSlot - kPage) return hipErrorOutOfMemory;
-  for (size_t s = 1; s < kSlots; s += 2) {  // every other slot: a free slot between any two allocations
+  // every other slot (a free slot between any two allocations); slots are not
+  // reused while unused ones remain, so a stale pointer keeps faulting
+  // (EMU_SLOT_REUSE=1: the lowest free slot first, as a caching allocator would)
+  static size_t next = 1;
+  static const bool reuse = getenv("EMU_SLOT_REUSE") && getenv("EMU_SLOT_REUSE")[0] == '1';
+  if (reuse) next = 1;
+  for (size_t tries = 0; tries < kSlots / 2; ++tries, next = next + 2 < kSlots ? next + 2 : 1) {
+    const size_t s = next;
     if (allocs()[s].live) continue;
     char* slot_end = base + (s + 1) * kSlot;
     char* start = slot_end - ((bytes + 15) & ~size_t(15));
@@ -386,6 +400,7 @@ hipError_t hipMalloc(void** p, size_t bytes) {
       return hipErrorOutOfMemory;
     memset(page, 0xCD, (size_t)(slot_end - page));  // device memory starts undefined
     allocs()[s] = Alloc{true, start, bytes, ++g_serial};
+    next = next + 2 < kSlots ? next + 2 : 1;
     *p = start;
     return hipSuccess;
   }
@@ -423,6 +438,12 @@ hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hip
     abort();
   }
   return hipMemcpy(d, s, n, k);
+}
+hipError_t hipMemcpy2D(void* d, size_t dpitch, const void* s, size_t spitch, size_t width, size_t height,
+                       hipMemcpyKind) {
+  for (size_t y = 0; y < height; ++y)
+    memcpy(static_cast<char*>(d) + y * dpitch, static_cast<const char*>(s) + y * spitch, width);
+  return hipSuccess;
 }
 hipError_t hipMemset(void* d, int v, size_t n) {
   if (n) memset(d, v, n);

@@ -158,3 +158,33 @@ def test_batch_rejects_mixed_widths_and_learned(renderer, scene):
         r.destroy_target(ta)
         r.destroy_target(tb)
         r.remove_mesh(s4)
+
+
+def test_batch_autotune_over_widths(renderer, scene):
+    """j3dg_autotune_batch times one graph launch of the batch for BVH width
+    {4, 8} x stack depth {8, 16, 32} x rays per work item {1, 2, 4} (the frames
+    given once per width), applies the fastest tuning and reports its width;
+    the batch at that setting still equals its frames rendered alone."""
+    r = renderer
+    mc, cav = J.make_matcap(0)
+    r.set_matcap(mc, cav)
+    s4 = [(r.upload_mesh(v, t, db_id=db, width=4), v, t, db) for _, v, t, db in scene]
+    frames8 = make_frames(scene, 7, seed=5)
+    frames4 = make_frames(s4, 7, seed=5)
+    targets = [r.create_target(fr.width, fr.height) for fr in frames8]
+    base = r.get_tuning()
+    try:
+        rep = r.autotune_batch(frames4, frames8, targets, repeats=2)
+        assert rep["width"] in (4, 8) and rep["n_tried"] == 18
+        assert r.get_tuning() == rep["tuning"]
+        chosen = frames8 if rep["width"] == 8 else frames4
+        expected = render_single(r, chosen, before=[r.read_target(tg)[0] for tg in targets])
+        r.render_batch(chosen, targets)
+        r.synchronize()
+        check(r, chosen, targets, expected, "autotuned batch")
+    finally:
+        r.set_tuning(**base)
+        for tg in targets:
+            r.destroy_target(tg)
+        for slot, _, _, _ in s4:
+            r.remove_mesh(slot)

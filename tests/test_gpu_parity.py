@@ -381,7 +381,7 @@ def test_autotune_keeps_results(renderer, tmp_path):
     cache = str(tmp_path / "tune.json")
     try:
         rep = renderer.autotune(fr, tg, repeats=2, cache=cache, key="torus@480x270")
-        assert rep["n_tried"] == 12 and not rep["cached"]
+        assert rep["n_tried"] == 12 and not rep["cached"] and rep["width"] == 8
         tun = renderer.get_tuning()
         assert tun == rep["tuning"]
         assert tun["stack_depth"] in (8, 16, 32) and tun["rays_per_item"] in (1, 2, 4)
@@ -397,6 +397,36 @@ def test_autotune_keeps_results(renderer, tmp_path):
         renderer.remove_mesh(slot)
     og, oim, _ = O.render([O.Mesh(v, t, db_id=J.make_db_id(1, 0))], fr, mc, cav)
     _assert_parity(g, im, og, oim, "autotuned torus 480x270")
+
+
+def test_autotune_chooses_width(renderer):
+    """j3dg_autotune_widths tunes the same frame over width-4 and width-8
+    uploads of its mesh (12 settings each), applies the faster tuning and
+    reports its width; the frame at the chosen width matches the oracle."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(300, 120, seed=11)
+    s4 = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=4)
+    s8 = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=8)
+    flags = EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT
+    f4 = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [s4], flags=flags, light_dir=light)
+    f8 = J.make_frame(480, 270, cam, cs, ci, pivot, diag, [s8], flags=flags, light_dir=light)
+    mc, cav = J.make_matcap(0)
+    renderer.set_matcap(mc, cav)
+    tg = renderer.create_target(480, 270)
+    base = renderer.get_tuning()
+    try:
+        rep = renderer.autotune_widths(f4, f8, tg, repeats=2)
+        assert rep["width"] in (4, 8) and rep["n_tried"] == 24
+        assert renderer.get_tuning() == rep["tuning"]
+        g, im = renderer.render(f8 if rep["width"] == 8 else f4, tg)
+        only8 = renderer.autotune_widths(None, f8, tg, repeats=2)
+        assert only8["width"] == 8 and only8["n_tried"] == 12
+    finally:
+        renderer.set_tuning(**base)
+        renderer.destroy_target(tg)
+        renderer.remove_mesh(s4)
+        renderer.remove_mesh(s8)
+    og, oim, _ = O.render([O.Mesh(v, t, db_id=J.make_db_id(1, 0))], f8, mc, cav)
+    _assert_parity(g, im, og, oim, "width-autotuned torus 480x270")
 
 
 @pytest.mark.parametrize("split", [0, 1])

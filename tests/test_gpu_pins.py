@@ -176,3 +176,30 @@ def test_render_scene_without_a_scene(renderer):
     exp["object_id"] = MISS
     assert sum(compare_gbuffers(g, exp).values()) == 0
     assert np.array_equal(im, O.background(w, h))
+
+
+def test_read_target_into_strided_rows(renderer):
+    """j3dg_read_target_strided: rows of the target's G-buffer and image land
+    `stride` elements apart (jtk::image rows, canvas.cpp:964-966), equal to
+    the packed readback, the padding untouched; a stride below the width is
+    refused."""
+    r = renderer
+    v, t = J.icosphere(3, seed=7, amplitude=0.05)
+    diag, pivot = J.scene_framing(v)
+    cs, ci = J.translation_camera(0.0, 0.0, 3.0)
+    mc, cav = J.make_matcap(0)
+    r.set_matcap(mc, cav)
+    slot = r.upload_mesh(v, t)
+    w, h = 37, 29
+    tg = r.create_target(w, h)
+    try:
+        fr = J.make_frame(w, h, J.camera_with_vertical_fov(60.0), cs, ci, pivot, diag, [slot])
+        g, im = r.render(fr, tg)
+        gs, ims = r.read_target_strided(tg, 40, 48)
+        assert sum(compare_gbuffers(gs[:, :w], g).values()) == 0 and np.array_equal(ims[:, :w], im)
+        assert np.all(gs["object_id"][:, w:] == 0) and np.all(ims[:, w:] == 0)
+        with pytest.raises(J.J3dgError):
+            r.read_target_strided(tg, w - 1, w)
+    finally:
+        r.destroy_target(tg)
+        r.remove_mesh(slot)
