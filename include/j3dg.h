@@ -322,6 +322,19 @@ int j3dg_trace_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, cons
                     uint32_t* obj, uint32_t* tri, float* t, float* u, float* v);
 int j3dg_any_hit_rays(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
                       uint8_t* hit);
+/* j3dg_trace_rays on DEVICE buffers, allocating nothing.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   rays: n x 8 floats on the context's device; obj, tri, t, u, v: n each on the
+   device (none may be NULL); results as j3dg_trace_rays.  At most
+   J3DG_MAX_OBJECTS slots.  Its working memory (queue words, counters, object
+   table, traversal-stack spill area) is one fixed 16 MiB array per GPU, held
+   until the process ends; a mesh set whose stack bound needs more spill than
+   it holds is refused (J3DG_ERR_UNSUPPORTED).  Synchronous.  Calls that
+   overlap on one GPU (from several contexts or threads) share that array and
+   race: serialise them. */
+int j3dg_trace_rays_device(j3dg_ctx* ctx, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
+                           uint32_t* obj, uint32_t* tri, float* t, float* u, float* v);
 
 /* All hits of each ray in one mesh (jtk::qbvh::find_all_triangles, called at j3d/vox.cpp:333): every
    triangle whose test accepts t in [t_near, t_far], in (t, triangle id) order.  counts[i] = the ray's

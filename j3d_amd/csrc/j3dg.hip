@@ -904,6 +904,18 @@ __global__ void k_clear_proposal(float2* p, uint64_t n) {
   if (i < n) p[i] = make_float2(0.f, -1.f);
 }
 
+// Working memory of j3dg_trace_rays_device, the ray query that allocates
+// nothing: one array per GPU, held until the process ends.  Layout (bytes):
+// [0, 1024) queue words, [1024, 2048) statistics counters, [2048, 8192) the
+// object table, [8192, end) the traversal-stack spill area.  Host code never
+// names `scratch` itself (in host code the name is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returned for the device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr size_t kNoAllocCounters = 1024, kNoAllocObjects = 2048, kNoAllocSpill = 8192;
+static_assert(kQueueWords * sizeof(uint32_t) <= kNoAllocCounters, "queue words fit their slot");
+static_assert(kNoAllocObjects + J3DG_MAX_OBJECTS * sizeof(DevObject) <= kNoAllocSpill, "object table fits its slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 
 // ============================================================= host side ===
 
@@ -1460,6 +1472,22 @@ extern "C" {
 
 int j3dg_abi_version(void) { return J3DG_ABI_VERSION; }
 
+// Device address of `scratch` on each GPU, looked up once per device (with
+// that device current) by the first j3dg_create on it.
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;
+
+static int noalloc_scratch_setup(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(j3d::scratch)) != hipSuccess || !p) return J3DG_ERR_LAUNCH;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return J3DG_OK;
+}
 
 int j3dg_device_count(int* count) {
   if (!count) return J3DG_ERR_INVALID_ARG;
@@ -1493,6 +1521,7 @@ int j3dg_create(int device, j3dg_ctx** out) {
     if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
   if (hipMalloc(&ctx->d_queue, kQueueWords * sizeof(uint32_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
   if (hipMalloc(&ctx->d_counters, 32 * sizeof(uint64_t)) != hipSuccess) return cleanup(J3DG_ERR_OUT_OF_MEMORY);
+  if (noalloc_scratch_setup(device, &ctx->noalloc_scratch) != J3DG_OK) return cleanup(J3DG_ERR_LAUNCH);
   *out = reinterpret_cast<j3dg_ctx*>(ctx);
   return J3DG_OK;
 }
@@ -2160,6 +2189,74 @@ int j3dg_any_hit_rays(j3dg_ctx* h, const uint32_t* slots, uint32_t n_slots, cons
   CHECK_CTX();
   if (!hit && n) return ctx->fail(J3DG_ERR_INVALID_ARG, "null output");
   return trace_rays_impl(ctx, slots, n_slots, rays, n, nullptr, nullptr, nullptr, nullptr, nullptr, hit, true);
+}
+
+// j3dg_trace_rays on device buffers with no allocation: the queue words,
+// counters, object table and stack spill area live in this GPU's `scratch`
+// array (noalloc_scratch, the address looked up at j3dg_create); the grid is
+// cut to what the spill area holds.  Calls that overlap on one GPU share that
+// array and race.
+int j3dg_trace_rays_device(j3dg_ctx* h, const uint32_t* slots, uint32_t n_slots, const float* rays, uint64_t n,
+                           uint32_t* obj, uint32_t* tri, float* t, float* u, float* v) {
+  Ctx* ctx = reinterpret_cast<Ctx*>(h);
+  CHECK_CTX();
+  if (!ctx->noalloc_scratch) return ctx->fail(J3DG_ERR_LAUNCH, "no scratch address for this device");
+  if (n_slots > J3DG_MAX_OBJECTS)
+    return ctx->fail(J3DG_ERR_INVALID_ARG, "j3dg_trace_rays_device takes at most J3DG_MAX_OBJECTS mesh slots");
+  if (n_slots && !slots) return ctx->fail(J3DG_ERR_INVALID_ARG, "null slot list");
+  if (n > 0xFFFFFFFFull / 2) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many rays in one call");
+  if (n == 0) return J3DG_OK;
+  if (!rays || !obj || !tri || !t || !u || !v) return ctx->fail(J3DG_ERR_INVALID_ARG, "null device buffer");
+  HIP_CALL(hipSetDevice(ctx->device), J3DG_ERR_NO_DEVICE);
+  DevObject tab[J3DG_MAX_OBJECTS];
+  std::memset(tab, 0, sizeof(tab));
+  uint32_t max_stack = 1;
+  int width = 4;
+  if (n_slots) {
+    int rc = ctx->fill_object_table(slots, n_slots, tab, max_stack, width);
+    if (rc) return rc;
+  }
+  const uint32_t D = ctx->tuning.stack_depth, block = ctx->tuning.block_size;
+  const uint32_t cap = max_stack > D ? max_stack - D + 1 : 1;
+  const uint64_t spill_entries = (kNoAllocScratchBytes - kNoAllocSpill) / sizeof(uint2);
+  const uint64_t blocks_fit = spill_entries / ((uint64_t)cap * block);
+  if (blocks_fit == 0)
+    return ctx->fail(J3DG_ERR_UNSUPPORTED, "the meshes' traversal-stack bound exceeds the scratch area (" +
+                                               std::to_string(kNoAllocScratchBytes >> 20) + " MiB)");
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ctx->trace_grid((uint32_t)n, D, block), blocks_fit);
+  unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
+  uint32_t* queue = reinterpret_cast<uint32_t*>(base);
+  uint64_t* counters = reinterpret_cast<uint64_t*>(base + kNoAllocCounters);
+  DevObject* objs = reinterpret_cast<DevObject*>(base + kNoAllocObjects);
+  uint2* spill = reinterpret_cast<uint2*>(base + kNoAllocSpill);
+  // queue heads and counters start at zero: a fill kernel, no memset node
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, queue, counters, 32u);
+  HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
+  if (n_slots)
+    HIP_CALL(hipMemcpyAsync(objs, tab, n_slots * sizeof(DevObject), hipMemcpyHostToDevice, ctx->stream),
+             J3DG_ERR_LAUNCH);
+  RayQuery q;
+  std::memset(&q, 0, sizeof(q));
+  q.rays = rays;
+  q.obj = obj, q.tri = tri, q.t = t, q.u = u, q.v = v;
+  FrameParams f;
+  std::memset(&f, 0, sizeof(f));
+  BatchParams bp;
+  std::memset(&bp, 0, sizeof(bp));
+  const DevObject* objs_c = objs;
+  uint32_t nr = (uint32_t)n, no = n_slots, capv = cap;
+  uint32_t batch = 64u * ctx->tuning.rays_per_item, refill = ctx->tuning.refill_min, coop = ctx->tuning.coop_tail;
+  uint64_t* tl = nullptr;
+  void* args[] = {&f, &q, &nr, &objs_c, &no, &spill, &capv, &queue, &counters, &batch, &refill, &coop, &tl, &bp};
+  void* fn = j3d::trace_kernel(kModeRays, width, (int)D, false);
+  HIP_CALL(hipLaunchKernel(fn, dim3(grid), dim3(block), args, D * block * sizeof(uint2), ctx->stream),
+           J3DG_ERR_LAUNCH);
+  uint64_t overflow = 0;
+  HIP_CALL(hipMemcpyAsync(&overflow, counters + 6, sizeof(overflow), hipMemcpyDeviceToHost, ctx->stream),
+           J3DG_ERR_LAUNCH);
+  HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
+  if (overflow) return ctx->fail(J3DG_ERR_INTERNAL, "traversal stack overflow");
+  return J3DG_OK;
 }
 
 int j3dg_calibration_copy(j3dg_ctx* h, uint64_t bytes, uint32_t repeats, float* ms) {

@@ -229,6 +229,7 @@ struct Ctx {
   std::vector<PointCloud> clouds;
   uint32_t* d_queue = nullptr;
   uint64_t* d_counters = nullptr;
+  void* noalloc_scratch = nullptr;  // this device's j3dg_trace_rays_device working memory (not owned)
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
   DevObject* d_objs = nullptr;   // the current object table (not owned: a ring slot or d_obj_big)

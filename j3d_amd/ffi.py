@@ -156,6 +156,7 @@ SIGNATURES = {
     "j3dg_render_batch": (_I, [_P, C.POINTER(FrameDesc), _P, _U32]),
     "j3dg_trace_rays": (_I, [_P, _P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     "j3dg_any_hit_rays": (_I, [_P, _P, _U32, _P, _U64, _P]),
+    "j3dg_trace_rays_device": (_I, [_P, _P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     "j3dg_calibration_copy": (_I, [_P, _U64, _U32, C.POINTER(_F)]),
     "j3dg_debug_timeline": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
     "j3dg_autotune": (_I, [_P, C.POINTER(FrameDesc), _U32, _U32, C.POINTER(AutotuneReport)]),
