@@ -132,6 +132,9 @@ struct Stack {
       }
       spill[(uint64_t)(e - D) * sstride] = lds[slot * stride];
     }
+    // lanes sharing one stack (the cooperative tail) all read the slot for
+    // the spill before any of them overwrites it; see pop
+    __builtin_amdgcn_wave_barrier();
     j3d_u32x2 v = {ref, __float_as_uint(dist)};
     lds[slot * stride] = v;
     return true;
@@ -144,6 +147,11 @@ struct Stack {
     --sp;
     const uint32_t slot = sp & (D - 1);
     const j3d_u32x2 v = lds[slot * stride];
+    // In the cooperative tail the lanes of a group pop one shared stack: each
+    // reads the top entry before any of them writes the spilled entry below it
+    // back into that slot.  The wave executes both in lockstep; the barrier
+    // (no instruction) keeps the compiler from moving the write above the read.
+    __builtin_amdgcn_wave_barrier();
     if (sp >= (uint32_t)D) lds[slot * stride] = spill[(uint64_t)(sp - D) * sstride];
     ref = v.x;
     dist = __uint_as_float(v.y);

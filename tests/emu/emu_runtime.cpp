@@ -228,6 +228,13 @@ dim3 block_idx() { return g_cur->block; }
 void* dyn_lds() { return g_lds.data() + g_cur->block_slot * g_lds_bytes; }
 void set_kernel_name(const char* name) { g_kernel = name; }
 
+bool live(const void* p, size_t bytes) {
+  const char* c = static_cast<const char*>(p);
+  if (!g_arena || c < g_arena || c >= g_arena + kSlot * kSlots) return false;
+  const Alloc& al = allocs()[(size_t)(c - g_arena) / kSlot];
+  return al.live && c >= al.start && c + bytes <= al.start + al.bytes;
+}
+
 void* device_array(size_t bytes) {
   void* p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) abort();
@@ -481,6 +488,15 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
 }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+  // EMU_RANDOM_TIME=<seed>: seeded random durations, so that choices made from
+  // timings (the autotuner's) take a different path on every seed
+  static const char* seed = getenv("EMU_RANDOM_TIME");
+  if (seed && seed[0]) {
+    static uint64_t x = 0x9E3779B97F4A7C15ull ^ strtoull(seed, nullptr, 10);
+    x ^= x << 13, x ^= x >> 7, x ^= x << 17;
+    *ms = 0.5f + (float)(x % 1000) / 1000.f;
+    return hipSuccess;
+  }
   *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
   return hipSuccess;
 }

@@ -115,6 +115,8 @@ extern dim3 g_block_dim, g_grid_dim;
 void* dyn_lds();
 // A fenced block of device memory for a file-scope __device__ array.
 void* device_array(size_t bytes);
+// Whether [p, p + bytes) lies inside one live device allocation (a debugging aid).
+bool live(const void* p, size_t bytes);
 void set_kernel_name(const char* name);
 
 // Runs every thread of every block of a launch: body() is the kernel call.
@@ -185,6 +187,11 @@ inline T shfl_impl(int kind, T v, int arg, const void* site, const void* frame) 
   ({ EMU_SITE_; ::emu::shfl_impl(::emu::kShflXor, (v), (int)(m), &emu_site_, __builtin_frame_address(0)); })
 #define __builtin_amdgcn_readfirstlane(v) \
   ({ EMU_SITE_; ::emu::shfl_impl(::emu::kFirstLane, (v), 0, &emu_site_, __builtin_frame_address(0)); })
+// The wave's lanes meet here (lockstep: every lane's earlier memory accesses
+// happen before any lane's later ones), e.g. a read and a write of one shared
+// LDS slot by the lanes of a group.
+#define __builtin_amdgcn_wave_barrier() \
+  do { EMU_SITE_; ::emu::wave_op(::emu::kAny, 0, 0, &emu_site_, __builtin_frame_address(0)); } while (0)
 #define __syncthreads() \
   do { EMU_SITE_; ::emu::wave_op(::emu::kBarrier, 0, 0, &emu_site_, __builtin_frame_address(0)); } while (0)
 inline uint32_t __builtin_amdgcn_mbcnt_lo(uint32_t m, uint32_t acc) {
