@@ -210,6 +210,75 @@ def orbit_sequence(r, tg, cam, pivot, diag, slot, flags, light, w, h, n, passes)
             "trace_ms_mean": float(np.mean(tm["trace_ms"]))}
 
 
+def bound_ceiling(r, tg, fr, slot, v, t, g, st, steps, warmup):
+    """What a perfect learned bound could save, for both ray kinds.
+
+    Primary rays: the plain frame's own hit distances written as the
+    proposal (margin 0; misses proposed as misses), rendered with the network
+    skipped -- the bounded pass at its best, with no inference cost; counted
+    and timed.  Shadow rays (any hit): rebuilt from the G-buffer, traced in
+    full (their node visits must equal the frame's), then the occluded ones
+    again inside a bound around their nearest occluder (closest-hit query):
+    [1e-3, t(1 + 1e-4)] and [t(1 - 1e-4), t(1 + 1e-4)].  An unoccluded ray
+    cannot be bounded (a miss must be proven over the whole ray)."""
+    import j3d_amd as J
+    from j3d_amd import ffi
+    from scenes import MISS, shadow_rays
+    hit = g["object_id"] != MISS
+    pairs = np.stack([np.where(hit, g["depth"], 0), np.where(hit, 1.0, -1.0)], axis=-1).astype(np.float32)
+    r.write_proposal(tg, pairs)
+    ref_g, _ = r.render(fr, tg)  # plain, over the same target contents as the bounded frame
+    pfr = J.Frame(fr.width, fr.height, fr.proj_inv, fr.cs, fr.cs_inv, fr.pivot, fr.diagonal, fr.objects,
+                  flags=fr.flags | ffi.LEARNED | ffi.KEEP_PROPOSAL, near_plane=fr.near_plane,
+                  light_dir=fr.light_dir, bg_top=fr.bg_top, bg_bottom=fr.bg_bottom, rect=fr.rect, margin=0.0)
+    r.set_counting(True)
+    pg, _ = r.render(pfr, tg)
+    pst = r.stats()
+    r.set_counting(False)
+    same = gbuffer_mismatches(pg, ref_g) == 0
+    for _ in range(warmup):
+        r.render(pfr, tg, readback=False)
+    r.synchronize()
+    r.reset_timings()
+    for _ in range(steps):
+        r.render(pfr, tg, readback=False)
+    r.synchronize()
+    perfect_trace_ms = float(np.mean(r.timings()["trace_ms"]))
+    out = {"primary": {"node_visits_plain": int(st["node_visits"]), "node_visits_perfect": int(pst["node_visits"]),
+                       "fallback": int(pst["learned_fallback"]), "gbuffer_equal_plain": bool(same),
+                       "trace_ms_perfect_no_network": perfect_trace_ms}}
+    rays, _ = shadow_rays(v, t, g, fr.light_dir)
+    r.set_counting(True)
+    occ = r.any_hit([slot], rays).astype(bool)
+    full = r.stats()
+    q = r.trace_rays([slot], rays[occ])
+    tocc = q["t"]
+    sh = {"rays": int(len(rays)), "occluded": int(occ.sum()), "frame_shadowed": int(st["shadowed"]),
+          "node_visits_frame": int(st["shadow_node_visits"]), "node_visits_full": int(full["node_visits"]),
+          "node_visits_occluded_full": None}
+    occ_rays = rays[occ]
+    r.any_hit([slot], occ_rays)
+    sh["node_visits_occluded_full"] = int(r.stats()["node_visits"])
+    for name, lo, hi in (("upper", None, 1 + 1e-4), ("interval", 1 - 1e-4, 1 + 1e-4)):
+        b = occ_rays.copy()
+        if lo is not None:
+            b[:, 6] = np.maximum(np.float32(1e-3), tocc * np.float32(lo))
+        b[:, 7] = tocc * np.float32(hi)
+        found = r.any_hit([slot], b).astype(bool)
+        sh["node_visits_occluded_" + name] = int(r.stats()["node_visits"])
+        sh["found_" + name] = int(found.sum())
+    r.set_counting(False)
+    unocc = sh["node_visits_full"] - sh["node_visits_occluded_full"]
+    sh["node_visits_perfect"] = unocc + min(sh["node_visits_occluded_upper"], sh["node_visits_occluded_interval"])
+    out["shadow"] = sh
+    plain = st["node_visits"] + st["shadow_node_visits"]
+    best = pst["node_visits"] + sh["node_visits_perfect"]
+    out["frame_node_visits_plain"] = int(plain)
+    out["frame_node_visits_perfect"] = int(best)
+    out["node_visit_ratio"] = best / max(1, plain)
+    return out
+
+
 def orbit_cameras(n, seed):
     """n orbit cameras looking at the origin: azimuth stratified over n equal
     sectors, elevation 15..55 degrees, distance 2.6..3.6, drawn from the
@@ -233,6 +302,12 @@ BUILDERS = {"host": 0, "device": 1}
 
 FUSED_PER_UNIT = ("%d B per node visit, 48 B per triangle test (primary and shadow), 32 B G-buffer write per pixel, "
                   "48 B hit completion record per hit")
+
+
+def gbuffer_mismatches(a, b):
+    """Values (bit patterns) that differ between two G-buffers, over every field."""
+    return int(sum(np.count_nonzero(a[k].view(np.uint32) != b[k].view(np.uint32)) if a[k].dtype.itemsize == 4
+                   else np.count_nonzero(a[k] != b[k]) for k in a.dtype.names))
 
 
 def fused_bytes(st, npx, node_bytes):
@@ -807,7 +882,7 @@ def splat_bench(a):
     fr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags, light_dir=light)
     P, _ = J.projection(cam, w, h)
     sflags = flags | ffi.SHADING
-    times = []
+    times, depth_ms = [], []
     for k in range(a.warmup + a.steps):
         r.render(fr, tg, readback=False)
         r.synchronize()
@@ -815,7 +890,9 @@ def splat_bench(a):
         r.splat_points(tg, P, ci, sflags, [cloud])
         if k >= a.warmup:
             times.append(time.perf_counter() - t1)
+            depth_ms.append(r.stats()["splat_ms"])
     ms = 1e3 * float(np.median(times))
+    kms = float(np.median(depth_ms))
     g0, im0 = r.render(fr, tg)
     r.splat_points(tg, P, ci, sflags, [cloud])
     g1, im1 = r.read_target(tg)
@@ -834,7 +911,9 @@ def splat_bench(a):
                "sample": "oracle render_pointclouds_on_image restatement, all %d points over the same frame, "
                          "%.1fs, one thread" % (n, cpu_s)}
         log("parity: %s; cpu %.1f M points/s" % (parity, cpu["value"]))
-    traffic = 12 * n  # xyz read once; keys, normals and colours of winners stay in the caches
+    # per point: its 12 B of xyz, read once, and the 8 B z-buffer key of its pixel (read, and
+    # written by an atomicMax only when it can grow); the keys (16.6 MB) stay in L2/MALL
+    point_bytes = 20 * n
     out = {"metric": "G points/s (56M-point cloud splatted over the 28M-triangle 1080p frame)",
            "value": n / (ms / 1e3) / 1e9, "unit": "G points/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -842,10 +921,12 @@ def splat_bench(a):
            "config": {"workload": "torus28m frame (1920x1080, shadows) + %d points (4 jittered copies of the "
                                   "torus vertices) with normals and colours, shading" % n, "points": n,
                       "width": w, "height": h},
-           "roofline": {"bound": "hbm", "achieved": traffic / (ms / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
-                        "frac": traffic / (ms / 1e3) / 1e9 / 8000.0, "traffic": None,
-                        "kernel": "whole j3dg_splat_points call (k_splat_init, k_splat_depth, k_splat_resolve)",
-                        "per_unit": "12 B xyz per point"},
+           "roofline": {"bound": "hbm", "achieved": point_bytes / (kms / 1e3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": point_bytes / (kms / 1e3) / 1e9 / 8000.0, "traffic": None,
+                        "kernel": "k_splat_depth (HIP events around it, median of the timed calls)",
+                        "launch_ms": kms,
+                        "per_unit": "12 B xyz + 8 B z-buffer key per point (the keys are a random access "
+                                    "served from L2/MALL: the HBM roofline is an upper bound it cannot reach)"},
            "parity": parity, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     r.close()
@@ -929,7 +1010,9 @@ def main():
 
     orbit = None
     if a.workload == "torus28m" and a.orbit_frames > 0:
-        orbit = orbit_sequence(r, tg, cam, pivot, diag, slot, flags, light, w, h, a.orbit_frames, a.steps)
+        otg = r.create_target(w, h)  # its own target: misses keep a target's earlier fields
+        orbit = orbit_sequence(r, otg, cam, pivot, diag, slot, flags, light, w, h, a.orbit_frames, a.steps)
+        r.destroy_target(otg)
         log("orbit: %s" % orbit)
 
     learned = None
@@ -946,23 +1029,28 @@ def main():
             log("autotune (learned): %s" % ltune)
             lfr = J.make_frame(w, h, cam, cs, ci, pivot, diag, [slot], flags=flags | ffi.LEARNED, light_dir=light,
                                rect=rect, margin=ltune["margin"])
+        # the reference: a plain frame over the same target contents (a miss keeps the
+        # target's earlier mark and barycentrics, j3d/canvas.cpp:859-866)
+        pg, pim = r.render(fr, tg)
         r.set_counting(True)
         lg, lim = r.render(lfr, tg)
         lst = r.stats()
         r.set_counting(False)
-        mism = int(sum(np.count_nonzero(lg[k].view(np.uint32) != gpu_g[k].view(np.uint32)) if lg[k].dtype.itemsize == 4
-                       else np.count_nonzero(lg[k] != gpu_g[k]) for k in lg.dtype.names))
+        mism = gbuffer_mismatches(lg, pg)
         lwall, ltm = timed(lfr)
         learned = {"mrays": rays_per_frame * a.steps / lwall / 1e6, "ms_per_step": lwall / a.steps * 1e3,
                    "mlp_ms": ltm["mlp_ms"], "trace_ms": ltm["trace_ms"], "shadow_stage_ms": ltm["shadow_ms"],
                    "margin": lfr.margin, "resolved": int(lst["learned_resolved"]),
                    "fallback": int(lst["learned_fallback"]), "hit_pixels": int(lst["hits"]),
                    "primary_node_visits": int(lst["node_visits"]), "plain_primary_node_visits": int(st["node_visits"]),
-                   "gbuffer_mismatches_vs_plain": mism, "rgba_equal_plain": bool(np.array_equal(lim, gpu_im)),
+                   "gbuffer_mismatches_vs_plain": mism, "rgba_equal_plain": bool(np.array_equal(lim, pim)),
                    "training": {k: rep_t[k] for k in ("steps", "loss_first", "loss_last", "train_ms", "label_ms",
                                                         "hit_fraction", "val_hit_accuracy", "val_mean_abs_error",
                                                         "val_resolved")},
                    "autotune": ltune}
+        if a.workload == "torus28m":
+            learned["perfect_bound_ceiling"] = bound_ceiling(r, tg, fr, slot, v, t, gpu_g, st, a.steps, a.warmup)
+            log("perfect-bound ceiling: %s" % learned["perfect_bound_ceiling"])
         ms_per_step, value = learned["ms_per_step"], learned["mrays"]
 
     node_bytes = 96 if a.width == 8 else 64  # quantized wide node

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define J3DG_ABI_VERSION 1
+#define J3DG_ABI_VERSION 2
 
 /* status codes */
 #define J3DG_OK 0
@@ -166,7 +166,9 @@ typedef struct j3dg_tuning {
                               there the three sorting launches cost more than the shorter tail saves */
 } j3dg_tuning;
 
-/* Statistics of the last j3dg_render / j3dg_render_batch. */
+/* Statistics of the last j3dg_render / j3dg_render_batch (with counting on,
+   also of the last j3dg_any_hit_rays: its work in node_visits, tri_tests,
+   wave_trips and lane_steps). */
 typedef struct j3dg_stats {
   uint64_t primary_rays, shadow_rays;
   uint64_t node_visits, tri_tests;   /* plain-kernel counters (only when counting is on) */
@@ -182,6 +184,7 @@ typedef struct j3dg_stats {
   float shade_ms;                    /* shading/edge kernel */
   float mlp_ms;                      /* learned proposal inference */
   float shadow_ms;                   /* k_finish + shadow-ray trace */
+  float splat_ms;                    /* k_splat_depth of the last j3dg_splat_points, summed over its clouds */
 } j3dg_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -263,7 +266,8 @@ int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32
    Times the trace stages of `frame` (median of `repeats` frames, default 5)
    over LDS stack depth {8,16,32} x rays per work item {1,2,4}, applies the
    fastest tuning to the context and, for a J3DG_LEARNED frame, also picks the
-   fastest bound margin (returned; the caller puts it in its frames).  The
+   fastest bound margin of {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8}
+   (returned; the caller puts it in its frames).  The
    report's width is that of the frame's meshes. */
 typedef struct j3dg_autotune_report {
   j3dg_tuning best;

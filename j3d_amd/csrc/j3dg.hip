@@ -40,6 +40,11 @@ enum : uint32_t {
 // Traversal steps a lane may take in one wave loop trip (3 measured best of
 // 1..16 on the 28M-triangle frame: 1 -> 0.87 ms, 2 -> 0.84, 3 -> 0.80, 4 -> 0.82).
 constexpr int kStepsPerTrip = 3;
+// A cooperative group (the tail with at most 8 rays left in the wave) takes up
+// to kCoopSteps steps per trip: no refill shares its trips any more (28M-
+// triangle frame: 1 -> 0.80 ms, 4 -> 0.78; 8 traversal steps per trip for the
+// tail's lanes outside groups, 0.79, and 16, 0.81, did not pay).
+constexpr int kCoopSteps = 4;
 
 // Lane phases of k_trace.
 enum : uint32_t { kIdle = 0u, kBounded = 1u, kPlain = 2u, kShadowPhase = 3u, kFinishPhase = 4u };
@@ -425,9 +430,13 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             }
           }
         }
+      } else if (coop) {
+        // a group's lanes share `done`: they leave the loop together
+        done = resume_top(coop_step<W, D, COUNT, kTop>(s, best, st, cc));
+        for (int extra = 1; extra < kCoopSteps && !done; ++extra)
+          done = resume_top(coop_step<W, D, COUNT, kTop>(s, best, st, cc));
       } else {
-        done = resume_top(coop ? coop_step<W, D, COUNT, kTop>(s, best, st, cc)
-                               : trav_step<W, D, COUNT, kTop>(s, best, st, cc));
+        done = resume_top(trav_step<W, D, COUNT, kTop>(s, best, st, cc));
       }
       if (!kTop && done && !(s.any && best.found) && s.obj + 1 < lnobj) {
         begin_object(s, lobjs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
@@ -945,7 +954,7 @@ void* trace_kernel(int mode, int width, int depth, bool count) {
   if (mode == (kModeFrame | kModeTop)) return trace_kernel_mode<kModeFrame | kModeTop>(width, depth, count);
   if (mode == kModeBatch) return trace_kernel_mode<kModeBatch>(width, depth, count);
   if (mode == kModeRays) return trace_kernel_mode<kModeRays>(width, depth, false);
-  if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, false);
+  if (mode == kModeAnyRays) return trace_kernel_mode<kModeAnyRays>(width, depth, count);
   if (mode == kModeShadow) return trace_kernel_mode<kModeShadow>(width, depth, count);
   if (mode == kModeFrameColor) return trace_kernel_mode<kModeFrameColor>(width, depth, count);
   return trace_kernel_mode<kModeFrame>(width, depth, count);
@@ -1516,6 +1525,8 @@ int j3dg_create(int device, j3dg_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
+  for (auto& e : ctx->splat_ev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
   ctx->ring.assign(8 * Ctx::kRing, nullptr);
   for (auto& e : ctx->ring)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(J3DG_ERR_LAUNCH);
@@ -1536,6 +1547,8 @@ void j3dg_destroy(j3dg_ctx* h) {
   for (auto& c : ctx->clouds) c.release();
   ctx->release_buffers();
   for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  for (auto& e : ctx->splat_ev)
     if (e) hipEventDestroy(e);
   for (auto& e : ctx->ring)
     if (e) hipEventDestroy(e);
@@ -1627,6 +1640,7 @@ int j3dg_get_stats(j3dg_ctx* h, j3dg_stats* st) {
   if (ctx->shaded && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) st->shade_ms = ms;
   if (ctx->mlp_ran && hipEventElapsedTime(&ms, e[4], e[5]) == hipSuccess) st->mlp_ms = ms;
   if (ctx->traced && hipEventElapsedTime(&ms, e[6], e[7]) == hipSuccess) st->shadow_ms = ms;
+  st->splat_ms = ctx->splat_ms;
   return J3DG_OK;
 }
 

@@ -208,18 +208,24 @@ int j3dg_splat_points(j3dg_ctx* h, uint32_t target, const float proj[16], const 
   std::memcpy(m.proj, proj, sizeof(m.proj));
   const bool one_bit = (flags & J3DG_ONE_BIT) != 0;
   const bool shade = (flags & J3DG_SHADING) != 0;
+  ctx->splat_ms = 0.f;
   for (uint32_t k = 0; k < n_clouds; ++k) {
     const PointCloud& c = ctx->clouds[clouds[k]];
     if (c.n == 0) continue;
     std::memcpy(m.cs, c.cs, sizeof(m.cs));
+    HIP_CALL(hipEventRecord(ctx->splat_ev[0], ctx->stream), J3DG_ERR_LAUNCH);
     hipLaunchKernelGGL(j3d::k_splat_depth, dim3(j3d::splat_grid(c.n, ctx->n_cus)), dim3(j3d::kSplatBlock), 0,
                        ctx->stream, c.d_xyz, c.n, m, tg.w, tg.h, keys);
     HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
+    HIP_CALL(hipEventRecord(ctx->splat_ev[1], ctx->stream), J3DG_ERR_LAUNCH);
     // canvas.cpp:994-995: normals only with shading, colours not in one-bit mode
     hipLaunchKernelGGL(j3d::k_splat_resolve, dim3(pix_blocks), dim3(j3d::kSplatBlock), 0, ctx->stream, keys,
                        (uint32_t)n_pix, shade ? c.d_normals : nullptr, one_bit ? nullptr : c.d_colors, m, shade,
                        c.db_id, tg.gbuf, tg.rgba);
     HIP_CALL(hipGetLastError(), J3DG_ERR_LAUNCH);
+    float ms = 0.f;
+    HIP_CALL(hipEventSynchronize(ctx->splat_ev[1]), J3DG_ERR_LAUNCH);
+    if (hipEventElapsedTime(&ms, ctx->splat_ev[0], ctx->splat_ev[1]) == hipSuccess) ctx->splat_ms += ms;
   }
   HIP_CALL(hipStreamSynchronize(ctx->stream), J3DG_ERR_LAUNCH);
   return J3DG_OK;

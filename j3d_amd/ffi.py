@@ -83,7 +83,8 @@ class Stats(C.Structure):
                 ("shadow_wave_trips", C.c_uint64), ("shadow_lane_steps", C.c_uint64),
                 ("max_ray_steps", C.c_uint64), ("max_shadow_ray_steps", C.c_uint64),
                 ("ray_steps_log2", C.c_uint64 * 7), ("shadow_ray_steps_log2", C.c_uint64 * 7),
-                ("trace_ms", C.c_float), ("shade_ms", C.c_float), ("mlp_ms", C.c_float), ("shadow_ms", C.c_float)]
+                ("trace_ms", C.c_float), ("shade_ms", C.c_float), ("mlp_ms", C.c_float), ("shadow_ms", C.c_float),
+                ("splat_ms", C.c_float)]
 
 
 class AutotuneReport(C.Structure):

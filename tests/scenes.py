@@ -90,3 +90,26 @@ def pick_crop(gbuf, side):
                 best = (d, x0, y0)
     _, x0, y0 = best
     return (x0, y0, x0 + side - 1, y0 + side - 1)
+
+
+def shadow_rays(v, t, g, light_dir):
+    """The directional-light shadow rays of a frame's hit pixels, rebuilt from
+    its G-buffer for a mesh drawn with the identity pose: origin = the hit
+    point interpolated from the triangle's vertices with the stored
+    barycentrics, in the order and float32 rounding finish_hit uses
+    (j3d/canvas.cpp:838-848), direction = the light, t in [1e-3, FLT_MAX].
+    Returns (rays n x 8, flat indices of the hit pixels)."""
+    flat = g.reshape(-1)
+    idx = np.nonzero(flat["object_id"] != MISS)[0]
+    tri = t[flat["object_id"][idx]]
+    bu = flat["barycentric_u"][idx].astype(np.float32)
+    bv = flat["barycentric_v"][idx].astype(np.float32)
+    s0 = (np.float32(1) - bu) - bv
+    v0, v1, v2 = (v[tri[:, k]].astype(np.float32) for k in range(3))
+    pos = (v0 * s0[:, None] + bu[:, None] * v1) + bv[:, None] * v2
+    rays = np.zeros((len(idx), 8), np.float32)
+    rays[:, 0:3] = pos
+    rays[:, 3:6] = np.asarray(light_dir, np.float32)[:3]
+    rays[:, 6] = 1e-3
+    rays[:, 7] = np.finfo(np.float32).max
+    return rays, idx

@@ -68,7 +68,7 @@ def test_record_layouts_match_the_c_compiler(tmp_path):
 
 def test_version_and_status_codes():
     lib = ffi.load()
-    assert lib.j3dg_abi_version() == 1
+    assert lib.j3dg_abi_version() == 2
     text = open(os.path.join(INC, "j3dg.h")).read()
     for code, name in ffi.ERRORS.items():
         assert re.search(r"#define J3DG_ERR_\w+ %d\b" % code, text), (code, name)

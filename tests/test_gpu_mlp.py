@@ -11,7 +11,7 @@ import pytest
 import j3d_amd as J
 import oracle as O
 from j3d_amd import ffi
-from scenes import c0_scene, compare_gbuffers, compare_rgba, torus_scene
+from scenes import c0_scene, compare_gbuffers, compare_rgba, shadow_rays, torus_scene
 from test_mlp_oracle import NORM, OFF, bf16, random_labels, random_params, random_rays
 
 pytestmark = pytest.mark.gpu
@@ -194,6 +194,52 @@ def test_perfect_proposal(renderer):
         s1["tri_tests"]))
     assert s1["node_visits"] <= s0["node_visits"]
     r.destroy_target(tg)
+
+
+def test_perfect_shadow_bound(renderer):
+    """Shadow rays rebuilt on the host from the G-buffer (the hit point from
+    the triangle's vertices and the stored barycentrics) reproduce the frame's
+    shadow bits through the any-hit query; inside a bound around each
+    occluded ray's nearest occluder (closest-hit query) every occluded ray is
+    still found, with fewer node visits -- the most a learned shadow bound
+    could save.  Prints the counts."""
+    r = renderer
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(600, 340, seed=3)
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(1, 10))
+    mc, cav = J.make_matcap(0)
+    r.set_matcap(mc, cav)
+    w, h = 640, 360
+    tg = r.create_target(w, h)
+    try:
+        fr = _frame(slot, w, h, cam, cs, ci, pivot, diag, light, FLAGS)
+        r.set_counting(True)
+        g, _ = r.render(fr, tg)
+        st = r.stats()
+        rays, idx = shadow_rays(v, t, g, fr.light_dir)
+        occ = r.any_hit([slot], rays).astype(bool)
+        full = r.stats()["node_visits"]
+        assert len(rays) == st["hits"] and occ.sum() == st["shadowed"]
+        assert np.array_equal(occ, (g.reshape(-1)["mark"][idx] & 1) == 1)
+        tocc = r.trace_rays([slot], rays[occ])["t"]
+        r.any_hit([slot], rays[occ])
+        occ_full = r.stats()["node_visits"]
+        b = rays[occ].copy()
+        b[:, 6] = np.maximum(np.float32(1e-3), tocc * np.float32(1 - 1e-4))
+        b[:, 7] = tocc * np.float32(1 + 1e-4)
+        found = r.any_hit([slot], b).astype(bool)
+        occ_bounded = r.stats()["node_visits"]
+        r.set_counting(False)
+        assert found.all()
+        # any-hit queries count their work like the frame's shadow phase (the
+        # order of a cooperative tail may differ, so the totals only nearly agree)
+        assert abs(full - st["shadow_node_visits"]) <= 0.1 * st["shadow_node_visits"]
+        assert 0 < occ_bounded <= occ_full < full
+        print("shadow rays %d, occluded %d; node visits: frame %d, query %d; occluded %d -> %d with a perfect bound"
+              % (len(rays), occ.sum(), st["shadow_node_visits"], full, occ_full, occ_bounded))
+    finally:
+        r.set_counting(False)
+        r.destroy_target(tg)
+        r.remove_mesh(slot)
 
 
 @pytest.fixture(scope="module")
