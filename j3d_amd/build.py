@@ -81,7 +81,7 @@ def build_canvas(verbose=False):
     if not os.path.exists(src):
         return None
     lib = os.path.join(LIBDIR, "libj3d_canvas.so")
-    _run([CXX] + COMMON + ["-pthread", "-shared", "-I" + HOST, "-o", lib, src,
+    _run([CXX] + COMMON + ["-pthread", "-shared", "-I" + HOST, "-o", lib, src, os.path.join(HOST, "ply.cpp"),
                            "-L" + LIBDIR, "-lj3dg", "-Wl,-rpath,$ORIGIN"])
     demo = os.path.join(HOST, "render_frame.cpp")
     if os.path.exists(demo):

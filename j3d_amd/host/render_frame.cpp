@@ -2,10 +2,11 @@
 // scene, renders it the way j3d's view does (view.cpp:421-430) and writes the
 // G-buffer (32 B per pixel) and RGBA image as raw little-endian files.
 //
-//   j3d_render_frame --scene icosphere LEVEL SEED AMP | --scene torus NU NV SEED
+//   j3d_render_frame --scene icosphere LEVEL SEED AMP | --scene torus NU NV SEED | --scene ply FILE
 //                    --size W H [--focal MM] [--cs 16 floats] [--flags edges,shadow,...]
 //                    [--light X Y Z] [--matcap 0..3] [--bg TOP BOTTOM] [--rect X0 Y0 X1 Y1]
 //                    [--frames N] [--learned MARGIN] [--copies N SPACING] [--points N SEED] [--device-build] [--out PREFIX]
+// --scene ply FILE: the mesh read by read_ply (j3d/io.cpp:733-736), as a viewer loads one
 // --copies: N objects of the mesh, object k translated by k * SPACING along x
 // --points N SEED: a point cloud of N points near the mesh, drawn after the shading
 // (db_id 2^29 + k).
@@ -20,6 +21,7 @@
 
 #include "canvas.h"
 #include "j3dg_scenegen.h"
+#include "ply.h"
 
 using namespace j3d_amd;
 
@@ -52,7 +54,7 @@ bool write_file(const std::string& path, const void* data, size_t bytes) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  std::string kind, flags = "edges,shading", out;
+  std::string kind, flags = "edges,shading", out, ply_file;
   uint32_t level = 7, nu = 0, nv = 0, w = 0, h = 0, frames = 1, mc = 0, copies = 1, n_points = 0;
   uint64_t point_seed = 0;
   float spacing = 0.f;
@@ -84,8 +86,11 @@ int main(int argc, char** argv) {
         nv = (uint32_t)std::strtoul(argv[i + 2], nullptr, 10);
         seed = std::strtoull(argv[i + 3], nullptr, 10);
         i += 3;
+      } else if (kind == "ply") {
+        need(1);
+        ply_file = argv[++i];
       } else {
-        usage("scene must be icosphere or torus");
+        usage("scene must be icosphere, torus or ply");
       }
     } else if (a == "--size") {
       need(2);
@@ -145,7 +150,11 @@ int main(int argc, char** argv) {
   try {
     std::vector<float> xyz;
     std::vector<uint32_t> tri;
-    if (kind == "icosphere") {
+    if (kind == "ply") {
+      std::vector<float> normals, uv;
+      std::vector<uint32_t> clrs;
+      if (!read_ply(ply_file.c_str(), xyz, normals, clrs, tri, uv)) usage(("read_ply: " + ply_error()).c_str());
+    } else if (kind == "icosphere") {
       xyz.resize(3 * j3dg_icosphere_vertex_count(level));
       tri.resize(3 * j3dg_icosphere_triangle_count(level));
       if (j3dg_gen_icosphere(level, seed, amp, xyz.data(), tri.data())) usage("icosphere generation failed");

@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from j3d_amd import ffi
+from j3d_amd import ffi, meshio
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
@@ -37,6 +37,18 @@ def test_every_declared_function_is_exported(header):
     assert not missing, missing
 
 
+def test_io_functions_are_exported_by_the_canvas_library():
+    """include/j3dg_io.h is the canvas layer's (libj3d_canvas.so), bound by j3d_amd.meshio."""
+    from j3d_amd import meshio
+    names = declared_functions("j3dg_io.h")
+    assert names
+    exp = exported(meshio.CANVAS_LIB)
+    assert not [n for n in names if n not in exp]
+    lib = meshio._load()
+    for n in names:
+        assert getattr(lib, n).argtypes is not None or getattr(lib, n).restype is not None, n
+
+
 def test_ctypes_binds_every_function():
     lib = ffi.load()
     names = set()
@@ -52,10 +64,12 @@ def test_record_layouts_match_the_c_compiler(tmp_path):
     structs = {"j3dg_pixel": None, "j3dg_frame_desc": ffi.FrameDesc, "j3dg_build_opts": ffi.BuildOpts,
                "j3dg_mesh_info": ffi.MeshInfo, "j3dg_tuning": ffi.Tuning, "j3dg_stats": ffi.Stats,
                "j3dg_train_opts": ffi.TrainOpts, "j3dg_train_report": ffi.TrainReport,
-               "j3dg_autotune_report": ffi.AutotuneReport, "j3dg_camera": ffi.Camera}
+               "j3dg_autotune_report": ffi.AutotuneReport, "j3dg_camera": ffi.Camera,
+               "j3dg_mesh_arrays": meshio.MeshArrays}
     src = tmp_path / "sizes.c"
     body = "\n".join('printf("%s %%zu\\n", sizeof(%s));' % (s, s) for s in structs)
-    src.write_text('#include <stdio.h>\n#include "j3dg.h"\n#include "j3dg_host.h"\nint main(void){%s return 0;}\n' % body)
+    src.write_text('#include <stdio.h>\n#include "j3dg.h"\n#include "j3dg_host.h"\n#include "j3dg_io.h"\n'
+                   'int main(void){%s return 0;}\n' % body)
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", "-I" + INC, "-o", str(exe), str(src)], check=True)
     sizes = dict(l.split() for l in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,

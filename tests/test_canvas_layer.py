@@ -24,7 +24,7 @@ def test_canvas_layer_built_and_linked():
     assert "libj3d_canvas.so" in out and "libj3dg.so" in out and "not found" not in out
 
 
-@pytest.mark.parametrize("args,msg", [([], "need --scene"), (["--scene", "cube"], "icosphere or torus"),
+@pytest.mark.parametrize("args,msg", [([], "need --scene"), (["--scene", "cube"], "icosphere, torus or ply"), (["--scene", "ply", "/nonexistent.ply", "--size", "8", "8"], "read_ply: cannot open"),
                                       (["--bogus"], "unknown argument")])
 def test_harness_rejects_bad_arguments(args, msg):
     r = subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
@@ -187,5 +187,41 @@ def test_canvas_points_in_scene_framing(tmp_path):
     og, oim = O.splat_points(og, oim, P, ci, flags, [{"xyz": rec[0], "normals": rec[1], "colors": col,
                                                       "db_id": (2 << 29) + 1}])
     assert (g["db_id"] == (2 << 29) + 1).sum() > 500
+    assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
+    assert compare_rgba(im, oim)[0] <= 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ascii", [False, True])
+def test_canvas_ply_scene_matches_oracle(tmp_path, ascii):
+    """A mesh loaded from a PLY file (read_ply, j3d/io.cpp:733-736) through the
+    canvas layer, framed by prepare_scene + unzoom: the frame equals the
+    oracle's over the arrays written to the file."""
+    from j3d_amd import meshio
+    w, h = 256, 192
+    cam = J.camera_with_vertical_fov(60.0)
+    v, t = J.torus(240, 130, seed=8)
+    path = str(tmp_path / ("mesh_a.ply" if ascii else "mesh_b.ply"))
+    rng = np.random.default_rng(3)
+    meshio.write_ply(path, v, t, normals=rng.normal(size=v.shape), colors=np.arange(len(v)), ascii=ascii)
+    prefix = str(tmp_path / "ply")
+    cmd = [EXE, "--scene", "ply", path, "--size", str(w), str(h), "--focal", repr(float(cam.focal_length)),
+           "--flags", "edges,shading", "--out", prefix]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = np.fromfile(prefix + ".gbuf", ffi.PIXEL_DTYPE).reshape(h, w)
+    im = np.fromfile(prefix + ".rgba", np.uint32).reshape(h, w)
+    ident = np.eye(4, dtype=np.float32).reshape(16)
+    lo, hi = J.bbox(v)
+    diag, pivot, cs, ci = O.prepare_and_unzoom([(ident, lo, hi)])
+    fr = J.Frame(w, h, J.projection(cam, w, h)[1], cs, ci, pivot, diag, [0], flags=ffi.EDGES | ffi.SHADING,
+                 near_plane=cam.near_plane)
+    mc, cav = O.matcap(0)
+    og = np.zeros((h, w), O.PIXEL_DTYPE)
+    og["object_id"] = 0xFFFFFFFF
+    og, _ = O.update_canvas([O.Mesh(v, t, db_id=1 << 29)], fr, gbuf=og)
+    oim = O.canvas_to_image(og, fr, mc, cav)
+    assert (g["object_id"] != 0xFFFFFFFF).sum() > 5000
     assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
     assert compare_rgba(im, oim)[0] <= 1
