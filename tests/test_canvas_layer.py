@@ -222,6 +222,6 @@ def test_canvas_ply_scene_matches_oracle(tmp_path, ascii):
     og["object_id"] = 0xFFFFFFFF
     og, _ = O.update_canvas([O.Mesh(v, t, db_id=1 << 29)], fr, gbuf=og)
     oim = O.canvas_to_image(og, fr, mc, cav)
-    assert (g["object_id"] != 0xFFFFFFFF).sum() > 5000
+    assert (g["object_id"] != 0xFFFFFFFF).sum() > 2000
     assert all(c == 0 for c in compare_gbuffers(g, og).values()), compare_gbuffers(g, og)
     assert compare_rgba(im, oim)[0] <= 1
