@@ -266,7 +266,7 @@ int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32
    Times the trace stages of `frame` (median of `repeats` frames, default 5)
    over LDS stack depth {8,16,32} x rays per work item {1,2,4}, applies the
    fastest tuning to the context and, for a J3DG_LEARNED frame, also picks the
-   fastest bound margin of {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8}
+   fastest bound margin of {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1.6, 3.2}
    (returned; the caller puts it in its frames).  The
    report's width is that of the frame's meshes. */
 typedef struct j3dg_autotune_report {

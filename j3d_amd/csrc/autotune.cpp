@@ -5,7 +5,7 @@
 // frames after one warm-up) over LDS stack depth {8, 16, 32} x rays per
 // work item {1, 2, 4}, keeps the fastest, then tries the refill threshold
 // {8, 16, 32, 48} there, and for J3DG_LEARNED frames then sweeps the bound
-// margin over {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8}.  BVH width {4, 8}: the same frame given twice, over width-4 and
+// margin over {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1.6, 3.2}.  BVH width {4, 8}: the same frame given twice, over width-4 and
 // width-8 uploads of its meshes (the caller owns the uploads), each tuned, the
 // faster kept.  Batches (j3dg_render_batch) are tuned on the time of their
 // one graph launch over width x depth x rays per work item.  The result
@@ -169,7 +169,7 @@ extern "C" int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32
   rep->best_margin = frame->margin;
   rep->width = frame_width(ctx, *frame);
   if (frame->flags & J3DG_LEARNED) {
-    const float margins[9] = {0.f, 0.005f, 0.01f, 0.02f, 0.05f, 0.1f, 0.2f, 0.4f, 0.8f};
+    const float margins[11] = {0.f, 0.005f, 0.01f, 0.02f, 0.05f, 0.1f, 0.2f, 0.4f, 0.8f, 1.6f, 3.2f};
     float bm = best_ms;
     for (float m : margins) {
       j3dg_frame_desc fd = *frame;

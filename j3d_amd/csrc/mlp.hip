@@ -299,12 +299,14 @@ __device__ inline void forward(const Packed* pk, const Weights& w, uint32_t lane
 
 // -------------------------------------------------------------- inference ---
 
+constexpr uint32_t kInferQueue = 128;  // ring of queued pixels per wave: < 32 left + 64 new fit
+
 // RAW: network outputs (y0, y1) of object-space rays (n x 8 floats).
 // Frame: per pixel of the rectangle, the bounded-pass tmax and hit logit over
 // the frame's objects that carry a network: (min of t_pred * (1 + margin),
 // max logit) over the objects predicting a hit; logit -1 if none does.
 template <bool RAW>
-__global__ __launch_bounds__(256) void k_infer(FrameParams f, const DevObject* __restrict__ objs, uint32_t n_obj,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_infer(FrameParams f, const DevObject* __restrict__ objs, uint32_t n_obj,
                                                const Packed* raw_pk, float4 raw_norm, const float* __restrict__ rays,
                                                uint32_t n_rays, float2* __restrict__ out) {
   const uint32_t lane0 = threadIdx.x & 63u;
@@ -313,6 +315,8 @@ __global__ __launch_bounds__(256) void k_infer(FrameParams f, const DevObject* _
   const uint32_t n_items = RAW ? n_rays : f.n_tiles * 64u;
   const uint32_t n_chunks = (n_items + 31u) / 32u;
   const float t_near = f.diagonal / 100.f;
+  // frames: per wave, a ring of pixel indices waiting for the network
+  __shared__ uint32_t q_lds[RAW ? 1 : 4][RAW ? 1 : kInferQueue];
   bool first = true;
   for (uint32_t o = 0; o < (RAW ? 1u : n_obj); ++o) {
     const Packed* pk = RAW ? raw_pk : reinterpret_cast<const Packed*>(objs[o].mlp);
@@ -326,44 +330,81 @@ __global__ __launch_bounds__(256) void k_infer(FrameParams f, const DevObject* _
     Weights w;
     load_weights(pk, lane0, w);
     uint32_t lane = lane0;
-    for (uint32_t c = wave; c < n_chunks; c += n_waves) {
-      // opaque per-trip lane index: keeps lane-dependent constants inside the loop
-      asm volatile("" : "+v"(lane));
-      const uint32_t j = lane & 31u, h = lane >> 5;
-      const uint32_t id = c * 32u + j;
-      bool valid = id < n_items;
-      f4 ro = {0.f, 0.f, 0.f, 1.f}, rd = {0.f, 0.f, 1.f, 0.f};
-      uint32_t px = 0, py = 0;
-      if (RAW) {
+    if constexpr (RAW) {
+      for (uint32_t c = wave; c < n_chunks; c += n_waves) {
+        // opaque per-trip lane index: keeps lane-dependent constants inside the loop
+        asm volatile("" : "+v"(lane));
+        const uint32_t j = lane & 31u;
+        const uint32_t id = c * 32u + j;
+        const bool valid = id < n_items;
+        f4 ro = {0.f, 0.f, 0.f, 1.f}, rd = {0.f, 0.f, 1.f, 0.f};
         if (valid) {
           const float* rr = rays + 8 * (size_t)id;
           ro = f4{rr[0], rr[1], rr[2], 1.f};
           rd = f4{rr[3], rr[4], rr[5], 0.f};
         }
-      } else if (valid) {
-        valid = tile_pixel(f, id >> 6, id & 63u, px, py);
-        if (valid) {
-          f4 wo, wd;
-          primary_ray(f, px, py, wo, wd);
-          ro = mat_vec(objs[o].inv_cs, wo);
-          rd = mat_vec(objs[o].inv_cs, wd);
-        }
+        float x[6], a;
+        ray_input(nrm, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, x, a);
+        float y0, y1;
+        forward(pk, w, lane, x, y0, y1);
+        if ((lane >> 5) == 0 && valid) out[id] = make_float2(y0, y1);
       }
-      float x[6], a;
-      const float dl = ray_input(nrm, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, x, a);
-      float y0, y1;
-      forward(pk, w, lane, x, y0, y1);
-      if (h == 0 && valid) {
-        if (RAW) {
-          out[id] = make_float2(y0, y1);
-        } else {
-          const size_t pix = (size_t)py * f.w + px;
-          const float tb = ((y0 * nrm[3] + a) / dl) * (1.f + f.margin);
-          const bool hit = y1 > 0.f && tb >= t_near && tb <= FLT_MAX;
-          float2 cur = first ? make_float2(0.f, -1.f) : out[pix];
-          if (hit) cur = cur.y > 0.f ? make_float2(fminf(cur.x, tb), fmaxf(cur.y, y1)) : make_float2(tb, y1);
-          out[pix] = cur;
+    } else {
+      // A ray that enters none of the object's root children boxes (the
+      // traversal's own first test) cannot hit the object: it gets no
+      // proposal from it (a miss proposal only sends a pixel to the plain
+      // pass, which is exact for any proposal).  The other rays queue up in
+      // LDS, 32 per network pass.
+      uint32_t* qw = q_lds[threadIdx.x >> 6];
+      uint32_t head = 0, tail = 0;  // wave-uniform
+      const float* ic = objs[o].inv_cs;
+      for (uint32_t b = wave;; b += n_waves) {
+        const bool more = b < f.n_tiles;  // a tile's 64 pixels per pass of this loop
+        if (more) {
+          uint32_t px, py;
+          bool need = false;
+          if (tile_pixel(f, b, lane0, px, py)) {
+            f4 wo, wd;
+            primary_ray(f, px, py, wo, wd);
+            const Ray r = make_object_ray(ic, wo, wd, t_near, FLT_MAX);
+            need = objs[o].width == 8 ? enters_node<8>(objs[o].nodes, r, FLT_MAX)
+                                      : enters_node<4>(objs[o].nodes, r, FLT_MAX);
+            if (!need && first) out[(size_t)py * f.w + px] = make_float2(0.f, -1.f);
+          }
+          const uint64_t m = __ballot(need);
+          if (need) qw[(tail + rank_in_mask(m)) & (kInferQueue - 1u)] = py * f.w + px;
+          tail += (uint32_t)__popcll(m);
         }
+        // the queue's writes reach LDS before any lane reads them back
+        __builtin_amdgcn_wave_barrier();
+        while (tail - head >= 32u || (!more && tail != head)) {
+          asm volatile("" : "+v"(lane));
+          const uint32_t j = lane & 31u;
+          const bool valid = head + j < tail;
+          const uint32_t pix = valid ? qw[(head + j) & (kInferQueue - 1u)] : 0u;
+          f4 ro = {0.f, 0.f, 0.f, 1.f}, rd = {0.f, 0.f, 1.f, 0.f};
+          if (valid) {
+            f4 wo, wd;
+            primary_ray(f, pix % f.w, pix / f.w, wo, wd);
+            ro = mat_vec(ic, wo);
+            rd = mat_vec(ic, wd);
+          }
+          float x[6], a;
+          const float dl = ray_input(nrm, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, x, a);
+          float y0, y1;
+          forward(pk, w, lane, x, y0, y1);
+          if ((lane >> 5) == 0 && valid) {
+            const float tb = ((y0 * nrm[3] + a) / dl) * (1.f + f.margin);
+            const bool hit = y1 > 0.f && tb >= t_near && tb <= FLT_MAX;
+            float2 cur = first ? make_float2(0.f, -1.f) : out[pix];
+            if (hit) cur = cur.y > 0.f ? make_float2(fminf(cur.x, tb), fmaxf(cur.y, y1)) : make_float2(tb, y1);
+            out[pix] = cur;
+          }
+          head += min(32u, tail - head);
+          // every lane has read its entry before later passes overwrite the ring
+          __builtin_amdgcn_wave_barrier();
+        }
+        if (!more) break;
       }
     }
     first = false;

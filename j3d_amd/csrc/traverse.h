@@ -269,6 +269,39 @@ __device__ inline uint32_t decode_node(const float4 (&v)[W == 8 ? 6 : 4], const 
   return dist[0] != INFINITY ? ref[0] : kNone;
 }
 
+// Whether the ray enters any child box of the wide node at np within
+// [t_near, thi] -- the first test a traversal makes at the root: a ray that
+// enters none of the root's children hits nothing in the object.
+template <int W>
+__device__ inline bool enters_node(const float4* __restrict__ np, const Ray& r, float thi) {
+  constexpr int kVec = W == 8 ? 6 : 4;
+  float4 v[kVec];
+#pragma unroll
+  for (int k = 0; k < kVec; ++k) v[k] = gload4(np + k);
+  const float tlo = r.t_near * kSlackDown;
+  const uint32_t ex = __float_as_uint(v[0].w);
+  float sc[3], bs[3];
+  sc[0] = __uint_as_float((ex & 0xFFu) << 23) * r.ix;
+  sc[1] = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.iy;
+  sc[2] = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.iz;
+  bs[0] = (v[0].x - r.ox) * r.ix;
+  bs[1] = (v[0].y - r.oy) * r.iy;
+  bs[2] = (v[0].z - r.oz) * r.iz;
+  float dist[W];
+#define U(x) __float_as_uint(x)
+  if constexpr (W == 4) {
+    test4(U(v[1].x), U(v[1].y), U(v[1].z), U(v[1].w), U(v[2].x), U(v[2].y), sc, bs, r, tlo, thi, dist);
+  } else {
+    test4(U(v[1].x), U(v[1].z), U(v[2].x), U(v[2].z), U(v[3].x), U(v[3].z), sc, bs, r, tlo, thi, dist);
+    test4(U(v[1].y), U(v[1].w), U(v[2].y), U(v[2].w), U(v[3].y), U(v[3].w), sc, bs, r, tlo, thi, dist + 4);
+  }
+#undef U
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < W; ++i) any |= dist[i] != INFINITY;
+  return any;
+}
+
 // Traversal state of one lane inside one object.
 struct Trav {
   Ray r;
