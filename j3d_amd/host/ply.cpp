@@ -257,6 +257,21 @@ bool read_ply(const char* filename, std::vector<float>& vertices, std::vector<fl
   for (const Elem& e : elems)
     if (e.name == "vertex") n_vertices = e.count;
   if (n_vertices > 0xFFFFFFFFull) return fail("too many vertices");
+  // every element instance takes at least one byte per property (binary: its
+  // size; ascii: a digit and a separator), so counts the file cannot hold are
+  // refused before anything is allocated for them
+  {
+    const uint64_t left = (uint64_t)(end - p);
+    uint64_t need = 0;
+    for (const Elem& e : elems) {
+      uint64_t rec = 0;
+      for (const Prop& pr : e.props)
+        rec += fmt == Fmt::ascii ? 2 : pr.list ? type_size(pr.count_ty) : type_size(pr.ty);
+      if (rec && e.count > left / rec) return fail("element " + e.name + ": more instances than the file holds");
+      need += e.count * rec;
+      if (need > left) return fail("the header describes more data than the file holds");
+    }
+  }
 
   // ---- data
   Source src(fmt, p, end);
@@ -463,7 +478,12 @@ int j3dg_read_ply(const char* filename, j3dg_mesh_arrays* out) {
   }
   std::vector<float> v, n, uv;
   std::vector<uint32_t> c, t;
-  if (!j3d_amd::read_ply(filename, v, n, c, t, uv)) return 1;
+  try {
+    if (!j3d_amd::read_ply(filename, v, n, c, t, uv)) return 1;
+  } catch (const std::exception& e) {
+    j3d_amd::g_error = std::string("read_ply: ") + e.what();
+    return 1;
+  }
   out->n_vertices = v.size() / 3;
   out->n_triangles = t.size() / 3;
   out->vertices = copy_out(v);
@@ -500,12 +520,17 @@ int j3dg_write_ply(const char* filename, const j3dg_mesh_arrays* m, int ascii) {
     return 1;
   }
   const size_t nv = (size_t)m->n_vertices, nt = (size_t)m->n_triangles;
-  std::vector<float> v(m->vertices, m->vertices + 3 * nv);
-  std::vector<float> n = m->normals ? std::vector<float>(m->normals, m->normals + 3 * nv) : std::vector<float>();
-  std::vector<uint32_t> c = m->colors ? std::vector<uint32_t>(m->colors, m->colors + nv) : std::vector<uint32_t>();
-  std::vector<uint32_t> t(m->triangles, m->triangles + 3 * nt);
-  std::vector<float> uv = m->uv ? std::vector<float>(m->uv, m->uv + 6 * nt) : std::vector<float>();
-  return j3d_amd::write_ply(filename, v, n, c, t, uv, ascii != 0) ? 0 : 1;
+  try {
+    std::vector<float> v(m->vertices, m->vertices + 3 * nv);
+    std::vector<float> n = m->normals ? std::vector<float>(m->normals, m->normals + 3 * nv) : std::vector<float>();
+    std::vector<uint32_t> c = m->colors ? std::vector<uint32_t>(m->colors, m->colors + nv) : std::vector<uint32_t>();
+    std::vector<uint32_t> t(m->triangles, m->triangles + 3 * nt);
+    std::vector<float> uv = m->uv ? std::vector<float>(m->uv, m->uv + 6 * nt) : std::vector<float>();
+    return j3d_amd::write_ply(filename, v, n, c, t, uv, ascii != 0) ? 0 : 1;
+  } catch (const std::exception& e) {
+    j3d_amd::g_error = std::string("write_ply: ") + e.what();
+    return 1;
+  }
 }
 
 const char* j3dg_ply_error(void) { return j3d_amd::g_error.c_str(); }

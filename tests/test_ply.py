@@ -165,6 +165,12 @@ def test_refusals(tmp_path):
     bad_tc = good.replace(b"end_header", b"property list uchar float texcoord\nend_header")
     with pytest.raises(meshio.PlyError, match="texcoord"):
         meshio.read_ply(_write(tmp_path / "c.ply", bad_tc + body + struct.pack("<B3iB4f", 3, 0, 1, 2, 4, 0, 0, 1, 1)))
+    huge = good.replace(b"element vertex 3", b"element vertex 4000000000")
+    with pytest.raises(meshio.PlyError, match="more instances than the file holds"):
+        meshio.read_ply(_write(tmp_path / "v.ply", huge + body))
+    many_faces = good.replace(b"element face 1", b"element face 1000")
+    with pytest.raises(meshio.PlyError, match="than the file holds"):
+        meshio.read_ply(_write(tmp_path / "m.ply", many_faces + body + struct.pack("<B3i", 3, 0, 1, 2)))
     with pytest.raises(meshio.PlyError, match="vertex 5 of 3"):
         meshio.write_ply(str(tmp_path / "w.ply"), [[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 5]])
     with pytest.raises(ValueError):
