@@ -393,6 +393,35 @@ def multi_camera(a):
     log("plain: %.1f Mrays/s, %.2f ms per step" % (plain_value, plain_ms))
     value, ms_per_step = plain_value, plain_ms
 
+    # every camera in one persistent k_trace + one k_shade_batch, captured in a HIP
+    # graph (j3dg_render_batch); each target holds its camera's plain frame, which
+    # the batch must reproduce exactly
+    r.render_batch(frames, targets)
+    r.synchronize()
+    bmism = 0
+    for (g0, im0), tg in zip(plain, targets):
+        g, im = r.read_target(tg)
+        bmism += sum(compare_gbuffers(g, g0).values()) + int(np.count_nonzero(im != im0))
+    del g, im
+    for _ in range(a.warmup):
+        r.render_batch(frames, targets)
+    r.synchronize()
+    r.reset_timings()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render_batch(frames, targets)
+    r.synchronize()
+    bwall = time.perf_counter() - t1
+    graph_ms = float(np.mean(r.timings()["trace_ms"]))
+    balgo = sum(fused_bytes(s, npx, 96) for s in pst)
+    batched = {"mrays": rays_per_step * a.steps / bwall / 1e6, "ms_per_step": bwall / a.steps * 1e3,
+               "graph_ms": graph_ms, "mismatches_vs_frames": bmism,
+               "roofline": {"bound": "hbm", "achieved": balgo / (graph_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                            "unit": "GB/s", "frac": balgo / (graph_ms * 1e-3) / HBM_PEAK,
+                            "kernel": "k_trace + k_shade_batch (one graph launch, all cameras)",
+                            "algorithmic_bytes_per_launch": int(balgo)}}
+    log("batched: %s" % batched)
+
     learned = None
     if not a.no_learned:
         n_train = a.train_rays or (1 << 25)
@@ -442,7 +471,7 @@ def multi_camera(a):
     if a.no_cpu_baseline or a.profile_only:
         r.close()
         out = {"metric": "Mrays/s @300M-tri %d cameras %dx%d (profile run: no oracle parity)" % (a.cameras, w, h),
-               "value": value, "plain_mrays": plain_value, "roofline": roof, "learned": learned}
+               "value": value, "plain_mrays": plain_value, "roofline": roof, "batched": batched, "learned": learned}
         print(json.dumps(out), flush=True)
         return
     # oracle parity on a crop of every camera, timed as the CPU baseline
@@ -489,13 +518,14 @@ def multi_camera(a):
     desc = {"workload": "displaced torus R=1 r=0.35, %dx%d grid, %d triangles, fBm 6 octaves; %d orbit cameras "
                         "%dx%d, primary + directional shadow rays, edges + matcap" % (nu, nv, len(t), a.cameras, w, h),
             "triangles": int(len(t)), "width": w, "height": h, "cameras": a.cameras, "shadow": "directional",
-            "launches": "one persistent k_trace + k_shade per camera (multi-frame batching is switched off)"}
+            "launches": "value: one k_trace + k_shade per camera (the learned pass does not batch); "
+                        "'batched': all cameras in one graph-captured k_trace + k_shade_batch"}
     out = {"metric": "Mrays/s @300M-tri %d cameras %dx%d + shadows: learned-bounded vs plain GPU traversal; "
                      "%% HBM peak" % (a.cameras, w, h),
            "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic", "config": desc, "roofline": roof,
-           "plain_mrays": plain_value, "plain_ms_per_step": plain_ms, "learned": learned,
+           "plain_mrays": plain_value, "plain_ms_per_step": plain_ms, "batched": batched, "learned": learned,
            "learned_over_plain": (learned["mrays"] / plain_value) if learned else None,
            "autotune_plain": tune, "rays_per_step": int(rays_per_step),
            "primary_rays": int(sum(s["primary_rays"] for s in pst)),
