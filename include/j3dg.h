@@ -259,7 +259,12 @@ int j3dg_read_target_strided(j3dg_ctx* ctx, uint32_t target, j3dg_pixel* gbuf, u
                              uint32_t* rgba, uint32_t rgba_stride);
 
 /* Many frames (cameras x meshes of uneven size) in one persistent launch,
-   captured in a HIP graph and replayed while the batch is unchanged. */
+   captured in a HIP graph and replayed while the batch is unchanged.  Each
+   frame equals the same frame rendered by j3dg_render into its target.  A
+   J3DG_LEARNED frame's proposals come from its meshes' networks, run before
+   the launch (or, with J3DG_KEEP_PROPOSAL, from its target as it stands).
+   Refused: frames with textures or vertex colours, object lists, or
+   J3DG_NO_SCENE; meshes of different BVH widths in one batch. */
 int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32_t* targets, uint32_t n);
 
 /* ---- autotuner (no counterpart in the reference) -------------------------

@@ -308,8 +308,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           tf = FLT_MAX;
           pix = py * F.w + px;
           phase = 2;
-          if (learned) {
-            const float2 pr = f.proposal[pix];
+          // a batch's learned frames carry their own proposal buffers
+          const float2* prop = M == kModeBatch ? F.proposal : learned ? f.proposal : nullptr;
+          if (prop) {
+            const float2 pr = prop[pix];
             if (pr.y > 0.f) {  // predicted hit: bounded pass first (pr.x = t_pred * (1 + margin))
               tf = pr.x;
               phase = 1;
@@ -485,7 +487,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           ev |= kEvFallback;
           phase = 2;
           st.sp = 0;
-          const float t0 = f.diagonal / 100.f, tb = s.r.t_far;
+          const float t0 = frame_ref<M>(f, bp, fi).diagonal / 100.f, tb = s.r.t_far;
           start_ray(tb >= t0 ? tb : t0, FLT_MAX);
         } else {
           if (phase == 1) ev |= kEvResolved;
@@ -2346,36 +2348,44 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
   std::vector<j3d::DevObject> objs;
   uint32_t max_stack = 1, max_w = 0, max_h = 0;
   int width = 0;
+  std::vector<uint32_t> infer;  // learned frames whose proposals the network computes
   for (uint32_t i = 0; i < n; ++i) {
     const j3dg_frame_desc& fd = frames[i];
     if (targets[i] >= ctx->targets.size() || !ctx->targets[targets[i]].live)
       return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown target in batch");
     Target& tg = ctx->targets[targets[i]];
     if (fd.width != tg.w || fd.height != tg.h) return ctx->fail(J3DG_ERR_INVALID_ARG, "frame size differs from target size");
-    if (fd.flags & (J3DG_LEARNED | J3DG_KEEP_PROPOSAL | J3DG_NO_SCENE))
-      return ctx->fail(J3DG_ERR_UNSUPPORTED, "batches take plain frames (no learned bound, no NO_SCENE)");
+    if (fd.flags & J3DG_NO_SCENE) return ctx->fail(J3DG_ERR_UNSUPPORTED, "NO_SCENE frames render one at a time");
     if (fd.n_objects > J3DG_MAX_OBJECTS) return ctx->fail(J3DG_ERR_INVALID_ARG, "too many objects");
     int rc = j3d::make_frame_params(fd, tg, fps[i]);
     if (rc) return ctx->fail(rc, "invalid frame in batch");
     fps[i].fused = 1;
-    fps[i].obj_base = (uint32_t)objs.size();
-    for (uint32_t k = 0; k < fd.n_objects; ++k) {
-      const uint32_t slot = fd.objects[k];
-      if (slot >= ctx->meshes.size() || !ctx->meshes[slot].live) return ctx->fail(J3DG_ERR_NOT_FOUND, "unknown mesh slot");
-      const Mesh& m = ctx->meshes[slot];
-      if (width && width != (int)m.info.width) return ctx->fail(J3DG_ERR_UNSUPPORTED, "a batch's meshes must share BVH width");
-      width = (int)m.info.width;
-      j3d::DevObject o;
-      std::memset(&o, 0, sizeof(o));
-      o.nodes = m.d_nodes, o.tris = m.d_tris, o.verts = m.d_verts, o.idx = m.d_idx, o.normals = m.d_normals;
-      o.fin = m.d_fin;
-      std::memcpy(o.cs, m.cs, sizeof(o.cs));
-      std::memcpy(o.inv_cs, m.inv_cs, sizeof(o.inv_cs));
-      o.db_id = m.db_id;
-      o.width = m.info.width;
-      o.vcol = m.d_vcol, o.uv = m.d_uv, o.tex = m.d_tex, o.tex_w = m.tex_w, o.tex_h = m.tex_h;
-      objs.push_back(o);
-      max_stack = std::max(max_stack, m.max_stack);
+    // the frame's objects, as a single frame's table holds them (networks included)
+    const size_t base = objs.size();
+    fps[i].obj_base = (uint32_t)base;
+    objs.resize(base + fd.n_objects);
+    if (fd.n_objects) std::memset(&objs[base], 0, fd.n_objects * sizeof(j3d::DevObject));
+    uint32_t fstack = 0;
+    int fwidth = 0;
+    rc = ctx->fill_object_table(fd.objects, fd.n_objects, objs.data() + base, fstack, fwidth);
+    if (rc) return rc;
+    if (fd.n_objects) {
+      if (width && width != fwidth) return ctx->fail(J3DG_ERR_UNSUPPORTED, "a batch's meshes must share BVH width");
+      width = fwidth;
+    }
+    max_stack = std::max(max_stack, fstack);
+    // learned frames: the bounded pass reads the target's proposal buffer,
+    // filled by the network before the launch (or kept, J3DG_KEEP_PROPOSAL)
+    if ((fd.flags & J3DG_LEARNED) && fd.n_objects) {
+      bool any_net = false;
+      for (uint32_t k = 0; k < fd.n_objects; ++k) any_net = any_net || ctx->meshes[fd.objects[k]].has_proposal;
+      const bool keep = (fd.flags & J3DG_KEEP_PROPOSAL) != 0;
+      if (keep || any_net) {
+        rc = ensure_proposal(ctx, tg);
+        if (rc) return rc;
+        fps[i].proposal = tg.proposal;
+        if (!keep) infer.push_back(i);
+      }
     }
     first[i + 1] = first[i] + fps[i].n_tiles;
     j3d::make_shade_params(fd, tg, *ctx, sps[i]);
@@ -2477,6 +2487,12 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
     HIP_CALL(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), J3DG_ERR_LAUNCH);
     ctx->graph_key.swap(kb);
   }
+  // the learned frames' proposals, before the graph (they change with the cameras)
+  for (size_t k = 0; k < infer.size(); ++k) {
+    const uint32_t i = infer[k];
+    rc = ctx->launch_mlp_frame(fps[i], fps[i].n_obj, ctx->d_bobjs + fps[i].obj_base, k == 0, k + 1 == infer.size());
+    if (rc) return rc;
+  }
   hipEvent_t* e = ctx->cur_ev();
   HIP_CALL(hipEventRecord(e[0], ctx->stream), J3DG_ERR_LAUNCH);
   HIP_CALL(hipGraphLaunch(ctx->graph_exec, ctx->stream), J3DG_ERR_LAUNCH);
@@ -2484,9 +2500,10 @@ int j3dg_render_batch(j3dg_ctx* h, const j3dg_frame_desc* frames, const uint32_t
   HIP_CALL(hipEventRecord(e[6], ctx->stream), J3DG_ERR_LAUNCH);
   HIP_CALL(hipEventRecord(e[7], ctx->stream), J3DG_ERR_LAUNCH);
   ctx->graph_launches++;
-  ctx->traced = true, ctx->shaded = false, ctx->mlp_ran = false;
+  ctx->traced = true, ctx->shaded = false, ctx->mlp_ran = !infer.empty();
   ctx->last_ev = e;
-  ctx->ring_flags[ctx->ring_n % Ctx::kRing] = 1u;  // trace time covers trace + shade
+  // trace time covers trace + shade; mlp time, every learned frame's network
+  ctx->ring_flags[ctx->ring_n % Ctx::kRing] = infer.empty() ? 1u : 5u;
   ctx->ring_n++;
   return J3DG_OK;
 }

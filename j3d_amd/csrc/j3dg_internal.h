@@ -287,7 +287,11 @@ struct Ctx {
   float top_delta(float origin_radius) const;
   int upload_object_table(const DevObject* tab, uint32_t n);
   // learned proposal for the frame's objects into fp.proposal (mlp.hip)
-  int launch_mlp_frame(const FrameParams& fp, uint32_t n_obj);
+  // the proposal network over a frame's pixels into fp.proposal; objs: the frame's
+  // object table (nullptr: the current single-frame table); the render's mlp
+  // events are recorded before the launch when `first`, after it when `last`
+  int launch_mlp_frame(const FrameParams& fp, uint32_t n_obj, const DevObject* objs = nullptr, bool first = true,
+                       bool last = true);
   int reset_counters();  // work-queue heads and statistics counters
   int launch_trace(const FrameParams& fp, uint32_t max_stack, int width, bool learned, Target* tg = nullptr);
   int launch_trace_kernel(int mode, const FrameParams& fp, const RayQuery& q, uint32_t n_rays, uint32_t n_obj,

@@ -820,16 +820,16 @@ void init_params(uint64_t seed, float s0, std::vector<float>& p) {
 
 }  // namespace
 
-int Ctx::launch_mlp_frame(const FrameParams& fp, uint32_t n_obj) {
+int Ctx::launch_mlp_frame(const FrameParams& fp, uint32_t n_obj, const DevObject* objs, bool first, bool last) {
   Ctx* ctx = this;
   hipEvent_t* e = cur_ev();
-  HIP_TRY(hipEventRecord(e[4], stream), J3DG_ERR_LAUNCH);
+  if (first) HIP_TRY(hipEventRecord(e[4], stream), J3DG_ERR_LAUNCH);
   const uint32_t grid = infer_grid(n_cus, fp.n_tiles * 64u);
-  hipLaunchKernelGGL(k_infer<false>, dim3(grid), dim3(256), 0, stream, fp, (const DevObject*)d_objs, n_obj,
+  hipLaunchKernelGGL(k_infer<false>, dim3(grid), dim3(256), 0, stream, fp, objs ? objs : (const DevObject*)d_objs, n_obj,
                      (const Packed*)nullptr, make_float4(0.f, 0.f, 0.f, 1.f), (const float*)nullptr, 0u,
                      const_cast<float2*>(fp.proposal));
   HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
-  HIP_TRY(hipEventRecord(e[5], stream), J3DG_ERR_LAUNCH);
+  if (last) HIP_TRY(hipEventRecord(e[5], stream), J3DG_ERR_LAUNCH);
   mlp_ran = true;
   return J3DG_OK;
 }

@@ -10,7 +10,7 @@ int device_build_mesh(Ctx* ctx, Mesh&, const float*, uint64_t, const uint32_t*, 
   return ctx->fail(J3DG_ERR_UNSUPPORTED, "the device BVH builder is not part of the CPU emulation");
 }
 
-int Ctx::launch_mlp_frame(const FrameParams&, uint32_t) {
+int Ctx::launch_mlp_frame(const FrameParams&, uint32_t, const DevObject*, bool, bool) {
   return fail(J3DG_ERR_UNSUPPORTED, "the proposal network is not part of the CPU emulation");
 }
 

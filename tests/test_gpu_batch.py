@@ -138,7 +138,7 @@ def test_batch_against_oracle_and_counters(renderer, scene):
             r.destroy_target(tg)
 
 
-def test_batch_rejects_mixed_widths_and_learned(renderer, scene):
+def test_batch_rejects_mixed_widths_and_no_scene(renderer, scene):
     r = renderer
     v, t = J.torus(30, 20, seed=5)
     s4 = r.upload_mesh(v, t, width=4)
@@ -151,13 +151,106 @@ def test_batch_rejects_mixed_widths_and_learned(renderer, scene):
     try:
         with pytest.raises(J.J3dgError):
             r.render_batch([fa, fb], [ta, tb])
-        fl = J.make_frame(64, 48, cam, cs, ci, pivot, diag, [scene[0][0]], flags=ffi.EDGES | ffi.LEARNED)
+        fn = J.make_frame(64, 48, cam, cs, ci, pivot, diag, [scene[0][0]], flags=ffi.EDGES | ffi.NO_SCENE)
         with pytest.raises(J.J3dgError):
-            r.render_batch([fl], [ta])
+            r.render_batch([fn], [ta])
     finally:
         r.destroy_target(ta)
         r.destroy_target(tb)
         r.remove_mesh(s4)
+
+
+def with_flags(fr, extra, margin=0.0):
+    return J.Frame(fr.width, fr.height, fr.proj_inv, fr.cs, fr.cs_inv, fr.pivot, fr.diagonal, fr.objects,
+                   flags=fr.flags | extra, near_plane=fr.near_plane, light_dir=fr.light_dir, bg_top=fr.bg_top,
+                   bg_bottom=fr.bg_bottom, rect=fr.rect, margin=margin)
+
+
+def adversarial_proposals(g, rng):
+    """Per pixel a bound of one kind: the plain hit distance, half of it (the
+    bounded pass finds nothing: fallback), far beyond it, negative, inf, NaN,
+    random, one ulp short; a fifth of the pixels predicted to miss."""
+    h, w = g.shape
+    depth = g["depth"]
+    kinds = rng.integers(0, 8, size=(h, w))
+    with np.errstate(over="ignore", invalid="ignore"):
+        tb = np.where(kinds == 0, depth, 0)
+        tb = np.where(kinds == 1, depth * 0.5, tb)
+        tb = np.where(kinds == 2, depth * 3.0 + 1.0, tb)
+        tb = np.where(kinds == 3, -1.0, tb)
+        tb = np.where(kinds == 4, np.inf, tb)
+        tb = np.where(kinds == 5, np.nan, tb)
+        tb = np.where(kinds == 6, rng.uniform(0, 10, size=(h, w)), tb)
+        tb = np.where(kinds == 7, np.nextafter(depth, np.float32(0)), tb)
+    logit = np.where(rng.random((h, w)) < 0.8, 1.0, -1.0)
+    return np.stack([tb, logit], axis=-1).astype(np.float32), logit
+
+
+def test_batch_learned_frames_with_kept_proposals(renderer, scene):
+    """Learned frames in a batch, each reading its own target's proposals
+    (J3DG_KEEP_PROPOSAL, every kind of bound): every frame equals its plain
+    frame rendered alone, and the bounded pass takes exactly the pixels of
+    the rectangles predicted to hit (resolved + fallback)."""
+    r = renderer
+    rng = np.random.default_rng(17)
+    frames = make_frames(scene, 10, seed=31)
+    plain = render_single(r, frames)
+    learned = [with_flags(fr, ffi.LEARNED | ffi.KEEP_PROPOSAL) for fr in frames]
+    targets = [r.create_target(fr.width, fr.height) for fr in frames]
+    try:
+        n_pred = 0
+        for fr, tg, (g, _) in zip(frames, targets, plain):
+            pairs, logit = adversarial_proposals(g, rng)
+            r.write_proposal(tg, pairs)
+            x0, y0, x1, y1 = fr.rect
+            n_pred += int((logit[y0:y1 + 1, x0:x1 + 1] > 0).sum())
+        r.render_batch(learned, targets)
+        r.synchronize()
+        check(r, frames, targets, plain, "learned batch")
+        st = r.stats()
+        assert st["learned_resolved"] + st["learned_fallback"] == n_pred
+        assert st["learned_fallback"] > 0 and st["learned_resolved"] > 0
+    finally:
+        for tg in targets:
+            r.destroy_target(tg)
+
+
+def test_batch_learned_frames_with_the_network(renderer):
+    """A mesh with a trained proposal network, four learned frames in one
+    batch: the network runs per frame before the graph launch (its time is
+    reported) and fills each target's proposals exactly as the same frame
+    rendered alone does; every frame equals its plain frame."""
+    r = renderer
+    v, t = J.torus(200, 100, seed=12)
+    slot = r.upload_mesh(v, t, db_id=J.make_db_id(4, 0))
+    targets = []
+    try:
+        r.train_proposal(slot, n_rays=1 << 21, epochs=2, batch=8192, seed=3)
+        mesh = [(slot, v, t, J.make_db_id(4, 0))]
+        frames = make_frames(mesh, 4, seed=41)
+        plain = render_single(r, frames)
+        learned = [with_flags(fr, ffi.LEARNED, margin=0.1) for fr in frames]
+        alone = []
+        for fr in learned:
+            tg = r.create_target(fr.width, fr.height)
+            r.render(fr, tg)
+            alone.append(r.read_proposal(tg))
+            r.destroy_target(tg)
+        assert sum(int((p[..., 1] > 0).sum()) for p in alone) > 0  # the network predicts hits
+        targets = [r.create_target(fr.width, fr.height) for fr in frames]
+        r.reset_timings()
+        r.render_batch(learned, targets)
+        r.synchronize()
+        check(r, frames, targets, plain, "learned batch (network)")
+        for i, (tg, p0) in enumerate(zip(targets, alone)):
+            assert np.array_equal(r.read_proposal(tg).view(np.uint32), p0.view(np.uint32)), i
+        st = r.stats()
+        assert st["learned_resolved"] + st["learned_fallback"] > 0
+        assert float(r.timings()["mlp_ms"][-1]) > 0.0
+    finally:
+        for tg in targets:
+            r.destroy_target(tg)
+        r.remove_mesh(slot)
 
 
 def test_batch_autotune_over_widths(renderer, scene):
