@@ -296,9 +296,9 @@ bool read_ply(const char* filename, std::vector<float>& vertices, std::vector<fl
       for (uint64_t i = 0; i < e.count; ++i) {
         for (size_t k = 0; k < e.props.size(); ++k) {
           if (e.props[k].list) {
-            if (!src.skip(e.props[k])) return fail("truncated vertex data");
+            if (!src.skip(e.props[k])) return fail("truncated or malformed vertex data");
           } else if (!src.read(e.props[k].ty, val[k])) {
-            return fail("truncated vertex data");
+            return fail("truncated or malformed vertex data");
           }
         }
         vertices[3 * i] = (float)val[ix], vertices[3 * i + 1] = (float)val[iy], vertices[3 * i + 2] = (float)val[iz];
@@ -325,11 +325,11 @@ bool read_ply(const char* filename, std::vector<float>& vertices, std::vector<fl
           const Prop& pr = e.props[k];
           if ((int)k == il || (int)k == it) {
             double cnt;
-            if (!src.read(pr.count_ty, cnt) || !(cnt >= 0) || cnt > 1e9) return fail("truncated face data");
+            if (!src.read(pr.count_ty, cnt) || !(cnt >= 0) || cnt > 1e9) return fail("truncated or malformed face data");
             const uint64_t n = (uint64_t)cnt;
             for (uint64_t j = 0; j < n; ++j) {
               double x;
-              if (!src.read(pr.ty, x)) return fail("truncated face data");
+              if (!src.read(pr.ty, x)) return fail("truncated or malformed face data");
               if ((int)k == il) {
                 if (!(x >= 0) || x >= (double)n_vertices)
                   return fail("face " + std::to_string(f) + " refers to vertex " + std::to_string((long long)x) +
@@ -340,7 +340,7 @@ bool read_ply(const char* filename, std::vector<float>& vertices, std::vector<fl
               }
             }
           } else if (!src.skip(pr)) {
-            return fail("truncated face data");
+            return fail("truncated or malformed face data");
           }
         }
         if (corners.size() < 3) continue;
@@ -362,7 +362,7 @@ bool read_ply(const char* filename, std::vector<float>& vertices, std::vector<fl
     } else {
       for (uint64_t i = 0; i < e.count; ++i)
         for (const Prop& pr : e.props)
-          if (!src.skip(pr)) return fail("truncated data in element " + e.name);
+          if (!src.skip(pr)) return fail("truncated or malformed data in element " + e.name);
     }
   }
   if (uv.empty() && !vuv.empty()) {

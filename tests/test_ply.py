@@ -158,6 +158,9 @@ def test_refusals(tmp_path):
     meshio.read_ply(_write(tmp_path / "ok.ply", good + body + struct.pack("<B3i", 3, 0, 1, 2)))
     with pytest.raises(meshio.PlyError, match="truncated"):
         meshio.read_ply(_write(tmp_path / "t.ply", good + body + struct.pack("<B2i", 3, 0, 1)))
+    bad_tok = b"ply\nformat ascii 1.0\nelement vertex 1\nproperty int x\nproperty int y\nproperty int z\nend_header\n1.5 0 0\n"
+    with pytest.raises(meshio.PlyError, match="malformed vertex data"):
+        meshio.read_ply(_write(tmp_path / "tok.ply", bad_tok))
     with pytest.raises(meshio.PlyError, match="refers to vertex 3"):
         meshio.read_ply(_write(tmp_path / "i.ply", good + body + struct.pack("<B3i", 3, 0, 1, 3)))
     with pytest.raises(meshio.PlyError, match="refers to vertex -1"):
