@@ -808,6 +808,11 @@ __global__ void k_no_scene(j3dg_pixel* g, uint64_t n) {
 constexpr uint32_t kOrderBins = 64;
 constexpr uint32_t kOrderBlock = 256;  // tiles per block of k_tile_bins / k_order_scatter
 constexpr uint32_t kOrderMinTiles = 8192;  // smallest rectangle (in 8x8 tiles) given a cost order
+// A tile holding a pixel of at least this many loop trips goes to the heaviest
+// bin whatever its sum, so that the frame's longest pixels start first (28M-
+// triangle 1080p frame: 0.784-0.789 ms without; threshold 24: 0.93, 32: 0.83,
+// 40: 0.753-0.778, 48: 0.761, 64: 0.777)
+constexpr uint32_t kLongPixelTrips = 48;
 
 __device__ inline uint32_t cost_bin(uint32_t c) {
   // 0: up to 2 trips per pixel (background); then four bins per doubling
@@ -831,14 +836,17 @@ __global__ void __launch_bounds__(kOrderBlock) k_tile_bins(const uint16_t* __res
   uint32_t bin = 0;
   if (i < n_tiles) {
     const uint4* p = reinterpret_cast<const uint4*>(cost + (size_t)i * 64);
-    uint32_t sum = 0;
+    uint32_t sum = 0, mx = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint4 q = p[k];
       sum += (q.x & 0xFFFFu) + (q.x >> 16) + (q.y & 0xFFFFu) + (q.y >> 16) + (q.z & 0xFFFFu) + (q.z >> 16) +
              (q.w & 0xFFFFu) + (q.w >> 16);
+      if (kLongPixelTrips)
+        mx = max(mx, max(max(max(q.x & 0xFFFFu, q.x >> 16), max(q.y & 0xFFFFu, q.y >> 16)),
+                         max(max(q.z & 0xFFFFu, q.z >> 16), max(q.w & 0xFFFFu, q.w >> 16))));
     }
-    bin = cost_bin(sum);
+    bin = kLongPixelTrips && mx >= kLongPixelTrips ? kOrderBins - 1u : cost_bin(sum);
     tile_bin[i] = (uint8_t)bin;
   }
   wave_bin_counts(bin, i < n_tiles, wc[threadIdx.x >> 6]);
