@@ -803,8 +803,11 @@ __global__ void k_no_scene(j3dg_pixel* g, uint64_t n) {
 // while the whole GPU is busy instead of forming the frame's tail).  A tile's
 // cost is the loop trips of its 64 pixels; tiles are binned by cost on a log2
 // scale (four bins per doubling, 64 bins) and laid out heaviest bin first, each bin in tile order so
-// that concurrently running waves stay on neighbouring tiles.  Three small
-// launches: per-block bin counts, a scan, a stable scatter.
+// that concurrently running waves stay on neighbouring tiles.  Two small
+// launches: per-block bin counts, then k_order_place (each block finds its
+// tiles' slots from its band's counts and scatters them stably); frames with
+// bands of more than kPlaceMaxBandBlocks blocks scan once (k_order_scan) and
+// scatter (k_order_scatter) instead.
 constexpr uint32_t kOrderBins = 64;
 constexpr uint32_t kOrderBlock = 256;  // tiles per block of k_tile_bins / k_order_scatter
 constexpr uint32_t kOrderMinTiles = 8192;  // smallest rectangle (in 8x8 tiles) given a cost order
@@ -912,6 +915,55 @@ __global__ void __launch_bounds__(kOrderBlock) k_order_scatter(const uint8_t* __
   __syncthreads();
   if (valid) {
     uint32_t pos = block_first[blockIdx.x * kOrderBins + bin];
+    for (uint32_t k = 0; k < w; ++k) pos += wc[k][bin];
+    order[pos + rank_in_mask(same)] = i;
+  }
+}
+
+// k_order_scan and k_order_scatter in one launch, for frames whose bands hold
+// at most kPlaceMaxBandBlocks blocks: each block sums its band's counts
+// itself (64 threads, one bin each) to find where its tiles of each bin go --
+// the band's first tile, then the heavier bins of the band, then the same bin
+// in the band's earlier blocks -- and scatters its tiles as k_order_scatter.
+constexpr uint32_t kPlaceMaxBandBlocks = 512;
+static_assert(kOrderBins == 64 && kOrderBlock >= 64, "one wave holds the bins");
+__global__ void __launch_bounds__(kOrderBlock) k_order_place(const uint8_t* __restrict__ tile_bin, uint32_t n_tiles,
+                                                              const uint32_t* __restrict__ block_counts, uint32_t nb,
+                                                              uint32_t* __restrict__ order) {
+  __shared__ uint32_t first_pos[kOrderBins];
+  __shared__ uint32_t wc[kOrderBlock / 64][kOrderBins];
+  const uint32_t blk = blockIdx.x;
+  uint32_t b = 0;
+  while (b + 1 < kBands && blk >= (b + 1) * nb / kBands) ++b;
+  const uint32_t b0 = b * nb / kBands, b1 = (b + 1) * nb / kBands;
+  if (threadIdx.x < kOrderBins) {  // wave 0: one bin per lane
+    const uint32_t bin = threadIdx.x;
+    uint32_t total = 0, before = 0;
+    for (uint32_t k = b0; k < b1; ++k) {
+      const uint32_t c = block_counts[k * kOrderBins + bin];
+      total += c;
+      before += k < blk ? c : 0u;
+    }
+    uint32_t suffix = total;  // the band's tiles in bins >= bin
+    for (uint32_t d = 1; d < kOrderBins; d <<= 1) {
+      const uint32_t o = (uint32_t)__shfl((int)suffix, (int)min(bin + d, kOrderBins - 1u), 64);
+      if (bin + d < kOrderBins) suffix += o;
+    }
+    first_pos[bin] = b0 * kOrderBlock + (suffix - total) + before;
+  }
+  const uint32_t i = blk * kOrderBlock + threadIdx.x;
+  const bool valid = i < n_tiles;
+  const uint32_t bin = valid ? tile_bin[i] : 0u;
+  const uint32_t w = threadIdx.x >> 6;
+  wave_bin_counts(bin, valid, wc[w]);
+  uint64_t same = 0;  // lanes of this wave with the same bin
+  for (uint32_t bb = 0; bb < kOrderBins; ++bb) {
+    const uint64_t mb = __ballot(valid && bin == bb);
+    if (bb == bin) same = mb;
+  }
+  __syncthreads();
+  if (valid) {
+    uint32_t pos = first_pos[bin];
     for (uint32_t k = 0; k < w; ++k) pos += wc[k][bin];
     order[pos + rank_in_mask(same)] = i;
   }
@@ -1334,9 +1386,14 @@ int Ctx::launch_trace(const FrameParams& fp_in, uint32_t max_stack, int width, b
     uint32_t* counts = tg->tile_cost + (fp.n_tiles + 3) / 4;
     hipLaunchKernelGGL(k_tile_bins, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint16_t*)tg->pixel_cost,
                        fp.n_tiles, bins, counts);
-    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, stream, nb, counts);
-    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint8_t*)bins, fp.n_tiles,
-                       (const uint32_t*)counts, tg->tile_order);
+    if ((nb + kBands - 1) / kBands <= kPlaceMaxBandBlocks) {
+      hipLaunchKernelGGL(k_order_place, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint8_t*)bins, fp.n_tiles,
+                         (const uint32_t*)counts, nb, tg->tile_order);
+    } else {
+      hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, stream, nb, counts);
+      hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, stream, (const uint8_t*)bins, fp.n_tiles,
+                         (const uint32_t*)counts, tg->tile_order);
+    }
     HIP_TRY(hipGetLastError(), J3DG_ERR_LAUNCH);
     tg->order_valid = true;
     tg->order_rect[0] = fp.x0, tg->order_rect[1] = fp.y0, tg->order_rect[2] = fp.x1, tg->order_rect[3] = fp.y1;
