@@ -157,7 +157,7 @@ typedef struct j3dg_tuning {
   uint32_t rays_per_item;  /* rays each lane takes per queue fetch: 1, 2 or 4 */
   uint32_t coop_tail;      /* 1 = once a wave's rays run out, its last <= 8 rays get 8 lanes each */
   uint32_t block_size;     /* threads per workgroup: 64, 128 or 256 */
-  uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves); default 32 */
+  uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves); default 16 */
   uint32_t split_passes;   /* 0 = one launch per frame (a lane traces its pixel's shadow ray right after
                               the primary hit); 1 = primary pass, hit completion, shadow pass */
   uint32_t cost_order;     /* 1 = hand out a target's tiles heaviest first, by the work each tile took in
