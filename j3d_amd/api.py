@@ -219,7 +219,9 @@ def write_proposal(path, params, norm):
     the float32 parameters (W1, b1, ..., W4, b4), all little-endian."""
     p = np.ascontiguousarray(params, "<f4")
     nrm = np.ascontiguousarray(norm, "<f4")
-    assert p.size == ffi.MLP_PARAMS and nrm.size == 4
+    if p.size != ffi.MLP_PARAMS or nrm.size != 4:
+        raise ValueError("proposal network: %d parameters and 4 normalisation values expected, got %d and %d"
+                         % (ffi.MLP_PARAMS, p.size, nrm.size))
     with open(path, "wb") as f:
         f.write(PROPOSAL_MAGIC)
         f.write(np.array([PROPOSAL_VERSION, len(PROPOSAL_LAYERS)] + list(PROPOSAL_LAYERS), "<u4").tobytes())
@@ -523,7 +525,9 @@ class Renderer:
             return
         p = np.ascontiguousarray(params, np.float32)
         nrm = np.ascontiguousarray(norm, np.float32)
-        assert p.size == ffi.MLP_PARAMS and nrm.size == 4
+        if p.size != ffi.MLP_PARAMS or nrm.size != 4:
+            raise ValueError("set_proposal: %d parameters and 4 normalisation values expected, got %d and %d"
+                             % (ffi.MLP_PARAMS, p.size, nrm.size))
         self._c(self.lib.j3dg_set_proposal(self.ctx, int(slot), ffi.ptr(p), ffi.ptr(nrm)), "j3dg_set_proposal")
 
     def save_proposal(self, slot, path):

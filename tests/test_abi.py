@@ -140,3 +140,36 @@ def test_build_lists_only_existing_sources():
     from j3d_amd import build as B
     for s in B.HIP_SOURCES + B.CPP_SOURCES:
         assert os.path.exists(os.path.join(B.CSRC, s)), s
+
+
+def test_proposal_checkpoint_round_trip_and_refusals(tmp_path):
+    """The proposal network's checkpoint (magic, version, layer sizes,
+    normalisation, parameters) reads back bit-for-bit; wrong parameter or
+    normalisation counts raise ValueError before anything is written, and a
+    foreign, truncated or re-versioned file is refused on reading."""
+    from j3d_amd import api
+    rng = np.random.default_rng(7)
+    p = rng.standard_normal(ffi.MLP_PARAMS).astype(np.float32)
+    nrm = np.array([0.5, -1.0, 2.0, 3.5], np.float32)
+    path = str(tmp_path / "net.bin")
+    api.write_proposal(path, p, nrm)
+    p2, nrm2 = api.read_proposal(path)
+    assert np.array_equal(p2, p) and np.array_equal(nrm2, nrm)
+    with pytest.raises(ValueError):
+        api.write_proposal(str(tmp_path / "short.bin"), p[:-1], nrm)
+    with pytest.raises(ValueError):
+        api.write_proposal(str(tmp_path / "norm.bin"), p, nrm[:3])
+    assert not os.path.exists(str(tmp_path / "short.bin"))
+    data = open(path, "rb").read()
+    for name, blob in (("magic", b"X" + data[1:]), ("cut", data[:-4]), ("long", data + b"\0" * 4),
+                       ("version", data[:8] + np.uint32(api.PROPOSAL_VERSION + 1).tobytes() + data[12:])):
+        bad = str(tmp_path / (name + ".bin"))
+        with open(bad, "wb") as f:
+            f.write(blob)
+        with pytest.raises(ValueError):
+            api.read_proposal(bad)
+    import j3d_amd as J
+    r = J.Renderer.__new__(J.Renderer)  # the check runs before any library call
+    r.ctx = None
+    with pytest.raises(ValueError):
+        r.set_proposal(0, p[:10], nrm)
