@@ -170,8 +170,11 @@ __device__ inline const FrameParams& frame_ref(const FrameParams& f, const Batch
     return f;
 }
 
+// At least 3 waves per SIMD (at most 168 VGPRs): the tail modes' code would
+// otherwise take the kernel just past that line, to 2 waves.  The counting
+// instances (diagnostics) keep whatever registers they need.
 template <int W, int D, bool COUNT, int MODE>
-__global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32_t n_rays_q,
+__global__ void __launch_bounds__(256, COUNT ? 1 : 3) k_trace(FrameParams f, RayQuery q, uint32_t n_rays_q,
                                                const DevObject* __restrict__ objs, uint32_t n_obj, uint2* spill,
                                                uint32_t spill_cap, uint32_t* queue, uint64_t* counters,
                                                uint32_t batch, uint32_t refill_min, uint32_t coop_tail,
@@ -214,6 +217,8 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
   uint32_t max_steps = 0, hist[7] = {0, 0, 0, 0, 0, 0, 0};
   uint32_t max_steps_s = 0, hist_s[7] = {0, 0, 0, 0, 0, 0, 0};  // fused shadow phase
   bool coop = false;                 // cooperative tail mode (wave-uniform)
+  bool split = false;                // its split form: a group's lanes traverse parts of the ray's tree
+  uint32_t gsz = 8;                  // lanes per group (wave-uniform; 2, 4 or 8 in split mode)
   // counting: per-wave timeline (shader clock at start, pool exhausted, tail mode, end; trips)
   uint64_t tl_start = COUNT ? (uint64_t)wall_clock64() : 0, tl_exh = 0, tl_coop = 0;
   uint32_t trips_coop = 0;
@@ -355,9 +360,14 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
     if (coop_tail && !coop && pool.exhausted) {
       const uint64_t act = __ballot(phase != 0);
       const uint32_t n_act = (uint32_t)__popcll(act);
-      if (n_act > 0 && n_act <= 8) {
-        // hand each remaining ray to a group of 8 lanes (group g: the g-th active lane)
-        const uint32_t g = lane_id() >> 3;
+      // the split tail (coop_tail 2, 3, 4) starts with 8, 16 or 32 rays left, in
+      // groups of 64 / (rays left rounded up to 8, 16, 32) lanes; the
+      // cooperative tail and scenes with a top level wait for 8 rays
+      const uint32_t most = (!kTop && coop_tail >= 2u) ? 8u << (coop_tail - 2u) : 8u;
+      if (n_act > 0 && n_act <= most) {
+        // hand each remaining ray to a group of gsz lanes (group g: the g-th active lane)
+        gsz = n_act <= 8u ? 8u : n_act <= 16u ? 4u : 2u;
+        const uint32_t g = lane_id() / gsz;
         uint64_t m = act;
         for (uint32_t i = 0; i < g && m; ++i) m &= m - 1;
         const bool has = g < n_act;
@@ -379,10 +389,21 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         shfl_from(st, leader);
         if (!has) phase = 0u;
         coop = true;
+        split = !kTop && coop_tail >= 2u;
+        if (split && has && split_lane(gsz) != 0u) {
+          // an empty stack in the column of a lane whose ray is complete: lanes
+          // 1 .. gsz-1 of group g take the free columns (gsz-1)g .. (gsz-1)g +
+          // gsz-2 in lane order (the n_act <= 64 / gsz active lanes leave at
+          // least (gsz - 1) n_act)
+          uint64_t fm = ~act;
+          for (uint32_t i = 0, k = g * (gsz - 1u) + split_lane(gsz) - 1u; i < k; ++i) fm &= fm - 1u;
+          st.init(lds_stack, spill, spill_cap, (threadIdx.x & ~63u) + (uint32_t)__builtin_ctzll(fm));
+          s.cur = kNone;
+        }
         if (COUNT) tl_coop = (uint64_t)wall_clock64();
       }
     }
-    const bool writer = !coop || group_lane() == 0;
+    const bool writer = !coop || (lane_id() & (gsz - 1u)) == 0u;
     if (COUNT) trips++;
     if (COUNT && coop) trips_coop++;
     if (phase != 0) {
@@ -415,6 +436,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             s.any = true;
             if (COUNT) ray_steps = 0;
             start_ray(1e-3f, FLT_MAX);
+            if (split && split_lane(gsz) != 0u) s.cur = kNone;  // split mode: lane 0 starts, the others take work
           }
         } else {
           if constexpr (W == 8) {
@@ -432,6 +454,10 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
             }
           }
         }
+      } else if (split) {
+        // a group's lanes share `done`: they leave the loop together
+        done = split_step<W, D, COUNT>(s, best, st, cc, gsz);
+        for (int extra = 1; extra < kCoopSteps && !done; ++extra) done = split_step<W, D, COUNT>(s, best, st, cc, gsz);
       } else if (coop) {
         // a group's lanes share `done`: they leave the loop together
         done = resume_top(coop_step<W, D, COUNT, kTop>(s, best, st, cc));
@@ -444,6 +470,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
         begin_object(s, lobjs, s.obj + 1, wo, wd, s.r.t_near, s.r.t_far, best);
         st.sp = 0;
         done = false;
+        if (split && split_lane(gsz) != 0u) s.cur = kNone;
       }
       if (done && COUNT && phase != kBounded && writer) {
         // per-ray step histograms (primary / shadow), kept in registers and reduced once at exit
@@ -489,6 +516,7 @@ __global__ void __launch_bounds__(256) k_trace(FrameParams f, RayQuery q, uint32
           st.sp = 0;
           const float t0 = frame_ref<M>(f, bp, fi).diagonal / 100.f, tb = s.r.t_far;
           start_ray(tb >= t0 ? tb : t0, FLT_MAX);
+          if (split && split_lane(gsz) != 0u) s.cur = kNone;
         } else {
           if (phase == 1) ev |= kEvResolved;
           phase = 0;
@@ -1655,6 +1683,7 @@ int j3dg_set_tuning(j3dg_ctx* h, const j3dg_tuning* t) {
     return ctx->fail(J3DG_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
   if (n.split_passes > 1) return ctx->fail(J3DG_ERR_INVALID_ARG, "split_passes must be 0 or 1");
   if (n.cost_order > 1) return ctx->fail(J3DG_ERR_INVALID_ARG, "cost_order must be 0 or 1");
+  if (n.coop_tail > 4) return ctx->fail(J3DG_ERR_INVALID_ARG, "coop_tail must be 0 .. 4");
   ctx->tuning = n;
   return J3DG_OK;
 }

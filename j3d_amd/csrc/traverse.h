@@ -114,10 +114,14 @@ struct Stack {
   bool overflow;
 
   __device__ void init(uint2* lds_base, uint2* spill_base, uint32_t spill_cap) {
-    lds = (lds_entry*)lds_base + threadIdx.x;
+    init(lds_base, spill_base, spill_cap, threadIdx.x);
+  }
+  // an empty stack in the LDS and spill column of thread `col` of the block
+  __device__ void init(uint2* lds_base, uint2* spill_base, uint32_t spill_cap, uint32_t col) {
+    lds = (lds_entry*)lds_base + col;
     stride = blockDim.x;
     sstride = (uint64_t)gridDim.x * blockDim.x;
-    spill = (glb_entry*)spill_base + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    spill = (glb_entry*)spill_base + ((uint64_t)blockIdx.x * blockDim.x + col);
     cap = spill_cap;
     sp = 0;
     overflow = false;
@@ -693,6 +697,99 @@ __device__ inline bool coop_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
     s.cur = coop_visit<W, D, COUNT>(s.nodes, s.cur, s.r, s.thi, st, c);
   }
   return s.cur == kNone ? pop_next<D, TOP>(s, st) : false;
+}
+
+// ---- split tail mode ----------------------------------------------------------
+// The alternative use of lane groups for the last rays of a wave: the lanes of
+// a group (gsz = 2, 4 or 8 of them: 64 / gsz groups for the wave's remaining
+// rays) each traverse their own part of the ray's tree with a stack of their
+// own, so a long ray's chain of dependent node fetches runs up to gsz ways in
+// parallel.  Group lane 0 keeps the ray's stack; the other lanes start idle,
+// each with an empty stack in the LDS and spill column of a lane whose ray is
+// complete.  After every step the idle lanes take the top entry of busy lanes'
+// stacks, and the group's best t becomes every lane's culling bound.  A lane's
+// hits stay its own until the group runs out of work; the group's result is
+// then the least hit in the order (t, object, triangle), the order the
+// single-lane traversal keeps, so it does not depend on which lane found what.
+// (Scenes with a top level keep the cooperative tail.)
+
+__device__ inline uint32_t split_lane(uint32_t gsz) { return lane_id() & (gsz - 1u); }
+// bits of the lanes of this lane's group in a wave-wide mask
+__device__ inline uint32_t split_mask(uint64_t m, uint32_t gsz) {
+  return (uint32_t)(m >> (lane_id() & ~(gsz - 1u))) & ((1u << gsz) - 1u);
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ inline uint32_t nth_set(uint32_t m, uint32_t k) {
+  for (uint32_t i = 0; i < k; ++i) m &= m - 1u;
+  return (uint32_t)__builtin_ctz(m);
+}
+
+// best := the group's least hit in (t, object, triangle) order, in every lane
+// (a hit found beats none).  Equal entries are the same hit: distinct hits of
+// one object differ in their triangle.
+__device__ inline void group_best(Hit& best, uint32_t gsz) {
+  for (uint32_t m = 1; m < gsz; m <<= 1) {
+    const float ot = __shfl_xor(best.t, (int)m, 64), ou = __shfl_xor(best.u, (int)m, 64);
+    const float ov = __shfl_xor(best.v, (int)m, 64);
+    const uint32_t otri = (uint32_t)__shfl_xor((int)best.tri, (int)m, 64);
+    const uint32_t oobj = (uint32_t)__shfl_xor((int)best.obj, (int)m, 64);
+    const bool ofound = __shfl_xor(best.found ? 1 : 0, (int)m, 64) != 0;
+    const bool take = ofound && (!best.found || ot < best.t ||
+                                 (ot == best.t && (oobj < best.obj || (oobj == best.obj && otri < best.tri))));
+    if (take) {
+      best.found = true;
+      best.t = ot, best.u = ou, best.v = ov, best.tri = otri, best.obj = oobj;
+    }
+  }
+}
+
+// One step of every lane of a group in split mode (called by all gsz lanes).
+// Returns true in every lane when the ray's traversal of the object is
+// complete -- every lane idle with an empty stack, or, for an any-hit ray, a
+// hit found by any lane -- and best then holds the group's result.
+template <int W, int D, bool COUNT>
+__device__ inline bool split_step(Trav& s, Hit& best, Stack<D>& st, Counters& c, uint32_t gsz) {
+  bool busy = s.cur != kNone;
+  if (busy && trav_step<W, D, COUNT>(s, best, st, c)) {
+    busy = false;  // stack run dry (or an any-hit ray's hit)
+    s.cur = kNone;
+  }
+  if (s.any && split_mask(__ballot(best.found), gsz)) {
+    group_best(best, gsz);
+    return true;
+  }
+  float bt = best.found ? best.t : INFINITY;
+  for (uint32_t m = 1; m < gsz; m <<= 1) bt = fminf(bt, __shfl_xor(bt, (int)m, 64));
+  if (bt < s.tmax) {
+    s.tmax = bt;
+    s.thi = bt * kSlackUp;
+  }
+  const uint32_t idle = split_mask(__ballot(!busy), gsz);
+  const uint32_t donors = split_mask(__ballot(busy && st.sp > 0u), gsz);
+  if (!idle) return false;
+  if (!donors) {
+    // all gsz lanes take part (a group's lanes change phase together); compared
+    // with the lanes present, so that a group could never wait on an absent one
+    if (idle != split_mask(__ballot(true), gsz)) return false;
+    group_best(best, gsz);
+    return true;
+  }
+  // the k-th idle lane takes the top entry of the k-th donor
+  const uint32_t j = split_lane(gsz), below = (1u << j) - 1u;
+  const uint32_t n = min((uint32_t)__popc(idle), (uint32_t)__popc(donors));
+  uint32_t ref = kNone;
+  float dist = INFINITY;
+  if (busy && st.sp > 0u && (uint32_t)__popc(donors & below) < n) st.pop(ref, dist);
+  int src = (int)lane_id();
+  if (!busy) {
+    const uint32_t k = (uint32_t)__popc(idle & below);
+    if (k < n) src = (int)((lane_id() & ~(gsz - 1u)) + nth_set(donors, k));
+  }
+  const uint32_t r2 = (uint32_t)__shfl((int)ref, src, 64);
+  const float d2 = __shfl(dist, src, 64);
+  if (!busy && src != (int)lane_id() && d2 <= s.thi) s.cur = r2;
+  return false;
 }
 
 // ---- scenes of more than J3DG_MAX_OBJECTS objects: a top level ------------

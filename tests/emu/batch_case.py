@@ -3,7 +3,7 @@ of libj3dg the J3DG_LIBRARY environment variable names (the CPU emulation in
 tests/test_emu_batch.py): frames of meshes of uneven size, several sizes,
 sub-rectangles and modes, rendered one at a time and then as one batch (graph
 capture and replays, then updated frame tables), compared bit for bit, and
-against the oracle.  Prints one line per check; exit status 0 when every
+against the oracle; EMU_TAIL picks the tail mode (the coop_tail tuning).  Prints one line per check; exit status 0 when every
 frame matches, 1 on a mismatch (a memory fault in the emulation exits 86)."""
 import math
 import os
@@ -58,6 +58,8 @@ def make_frames(ms, n, seed):
 def main():
     n_frames = int(sys.argv[1]) if len(sys.argv) > 1 else 13
     r = J.Renderer(0)
+    if os.environ.get("EMU_TAIL"):  # the tail mode (coop_tail tuning) to run the batches with
+        r.set_tuning(**dict(r.get_tuning(), coop_tail=int(os.environ["EMU_TAIL"])))
     mc, cav = J.make_matcap(0)
     r.set_matcap(mc, cav)
     ms = meshes(r)

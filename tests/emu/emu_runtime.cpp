@@ -380,7 +380,10 @@ hipError_t hipGetDevice(int* d) {
 }
 hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
   memset(p, 0, sizeof(*p));
-  p->multiProcessorCount = 1;  // a persistent grid of a few blocks
+  // a persistent grid of a few blocks; EMU_CUS=n makes it n times as many
+  // (more waves, each holding fewer rays: every wave reaches its tail sooner)
+  static const int cus = getenv("EMU_CUS") ? atoi(getenv("EMU_CUS")) : 1;
+  p->multiProcessorCount = cus > 0 && cus <= 64 ? cus : 1;
   strcpy(p->gcnArchName, "gfx950-cpu-emulation");
   strcpy(p->name, "CPU emulation");
   p->totalGlobalMem = kSlot * kSlots;

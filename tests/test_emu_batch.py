@@ -9,6 +9,9 @@ fell outside of) and driven through the same C ABI by tests/emu/batch_case.py:
 rendered alone and as a batch (capture, replay, updated frame tables), each
 batch frame equal to its single render and to the oracle.
 
+tests/emu/tail_case.py runs frames of one and of three posed objects and ray
+queries with each tail mode (the coop_tail tuning) against the oracle.
+
 The two mistakes that faulted the GPU with an illegal memory access are
 planted in copies of the source and must fault here too, while the current
 source runs clean:
@@ -44,17 +47,31 @@ def libs(tmp_path_factory):
         return dict(ex.map(one, VARIANTS))
 
 
-def _run(lib):
-    env = dict(os.environ, J3DG_LIBRARY=lib)
-    return subprocess.run([sys.executable, os.path.join(EMU, "batch_case.py")], env=env, stdout=subprocess.PIPE,
+def _run(lib, case="batch_case.py", **env_extra):
+    env = dict(os.environ, J3DG_LIBRARY=lib, **env_extra)
+    return subprocess.run([sys.executable, os.path.join(EMU, case)], env=env, stdout=subprocess.PIPE,
                           stderr=subprocess.STDOUT, text=True, timeout=600)
 
 
-def test_batch_on_the_cpu_equals_single_frames_and_oracle(libs):
-    r = _run(libs[None])
+@pytest.mark.parametrize("tail", ["1", "3"])
+def test_batch_on_the_cpu_equals_single_frames_and_oracle(libs, tail):
+    # tail 3 (the default) on a grid of many waves (EMU_CUS): every wave ends in the split tail
+    r = _run(libs[None], EMU_TAIL=tail, **({"EMU_CUS": "16"} if tail == "3" else {}))
     print(r.stdout[-3000:])
     assert r.returncode == 0 and "BATCH OK" in r.stdout, r.stdout[-3000:]
     assert "shuffles that read a lane outside the exec mask: 0" in r.stdout
+
+
+@pytest.mark.parametrize("tail,depth", [("3", "8"), ("4", "32"), ("2", "16"), ("1", "8")])
+def test_tail_modes_on_the_cpu_match_the_oracle(libs, tail, depth):
+    """Frames of one and of three posed objects and closest-/any-hit queries
+    with each tail mode: the split tail from 16, 32 and 8 rays left (groups of
+    4, 2 and 8 lanes traversing parts of a ray's tree on stacks of their own,
+    in columns of finished lanes; depth 8 spills to global memory) and the
+    cooperative one, against the oracle."""
+    r = _run(libs[None], "tail_case.py", EMU_TAIL=tail, EMU_STACK=depth, EMU_CUS="16")
+    print(r.stdout[-3000:])
+    assert r.returncode == 0 and "TAIL OK" in r.stdout, r.stdout[-3000:]
 
 
 @pytest.mark.parametrize("variant", ["wrong_table", "no_frame_handoff"])
