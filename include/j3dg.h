@@ -158,8 +158,9 @@ typedef struct j3dg_tuning {
   uint32_t coop_tail;      /* once a wave's rays run out, its last rays get groups of lanes: 0 = off;
                               1 = the last <= 8 rays get 8 lanes each, which split each node's children;
                               2, 3, 4 = from 8, 16 or 32 rays left, each ray gets 64 / (8, 16 or 32)
-                              lanes, which traverse parts of its tree, each with a stack of its own
-                              (scenes with a top level use 1 for any value >= 1); default 3 */
+                              lanes, which traverse parts of its tree, each with a stack of its own;
+                              as rays finish the rest regroup into wider groups, up to 8 lanes
+                              (scenes with a top level use 1 for any value >= 1); default 4 */
   uint32_t block_size;     /* threads per workgroup: 64, 128 or 256 */
   uint32_t refill_min;     /* idle lanes of a wave that trigger handing out new rays: 1..64 (64 = whole waves); default 16 */
   uint32_t split_passes;   /* 0 = one launch per frame (a lane traces its pixel's shadow ray right after

@@ -378,7 +378,7 @@ class Renderer:
     def synchronize(self):
         self._c(self.lib.j3dg_synchronize(self.ctx), "j3dg_synchronize")
 
-    def set_tuning(self, stack_depth=16, rays_per_item=1, coop_tail=3, block_size=256, refill_min=16, split_passes=0,
+    def set_tuning(self, stack_depth=16, rays_per_item=1, coop_tail=4, block_size=256, refill_min=16, split_passes=0,
                    cost_order=1):
         t = ffi.Tuning(stack_depth, rays_per_item, coop_tail, block_size, refill_min, split_passes, cost_order)
         self._c(self.lib.j3dg_set_tuning(self.ctx, C.byref(t)), "j3dg_set_tuning")

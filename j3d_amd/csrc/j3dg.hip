@@ -272,6 +272,31 @@ __global__ void __launch_bounds__(256, COUNT ? 1 : 3) k_trace(FrameParams f, Ray
     return top_l ? top_next<D>(s, st, lobjs, top_l, f.top_delta, wo, wd)
                  : top_next<D>(s, st, lobjs, top, f.top_delta, wo, wd);
   };
+  // the tail modes: this lane takes over lane src's ray (all of its state)
+  auto take_state = [&](int src) {
+    shfl_from(phase, src);
+    shfl_from(pix, src);
+    shfl_from(ray_steps, src);
+    shfl_from(cost_slot, src);
+    shfl_from(pix_trips, src);
+    if (M == kModeBatch) {  // the ray's frame travels with it
+      shfl_from(fi, src);
+      shfl_from(lnobj, src);
+      shfl_from(lobjs, src);
+    }
+    shfl_from(wo, src);
+    shfl_from(wd, src);
+    shfl_from(s, src);
+    shfl_from(best, src);
+    shfl_from(st, src);
+  };
+  // split mode: a lane that starts idle, with an empty stack in the column of
+  // the k-th lane of `free_cols` (lanes of this wave whose columns nobody owns)
+  auto fresh_helper = [&](uint64_t free_cols, uint32_t k) {
+    for (uint32_t i = 0; i < k; ++i) free_cols &= free_cols - 1u;
+    st.init(lds_stack, spill, spill_cap, (threadIdx.x & ~63u) + (uint32_t)__builtin_ctzll(free_cols));
+    s.cur = kNone;
+  };
 
   for (;;) {
     uint32_t ev = 0;
@@ -371,36 +396,43 @@ __global__ void __launch_bounds__(256, COUNT ? 1 : 3) k_trace(FrameParams f, Ray
         uint64_t m = act;
         for (uint32_t i = 0; i < g && m; ++i) m &= m - 1;
         const bool has = g < n_act;
-        const int leader = has ? (int)__builtin_ctzll(m) : (int)lane_id();
-        shfl_from(phase, leader);
-        shfl_from(pix, leader);
-        shfl_from(ray_steps, leader);
-        shfl_from(cost_slot, leader);
-        shfl_from(pix_trips, leader);
-        if (M == kModeBatch) {  // the ray's frame travels with it
-          shfl_from(fi, leader);
-          shfl_from(lnobj, leader);
-          shfl_from(lobjs, leader);
-        }
-        shfl_from(wo, leader);
-        shfl_from(wd, leader);
-        shfl_from(s, leader);
-        shfl_from(best, leader);
-        shfl_from(st, leader);
-        if (!has) phase = 0u;
+        take_state(has ? (int)__builtin_ctzll(m) : (int)lane_id());
+        if (!has) phase = 0u, pix_trips = 0u;
         coop = true;
         split = !kTop && coop_tail >= 2u;
-        if (split && has && split_lane(gsz) != 0u) {
-          // an empty stack in the column of a lane whose ray is complete: lanes
-          // 1 .. gsz-1 of group g take the free columns (gsz-1)g .. (gsz-1)g +
-          // gsz-2 in lane order (the n_act <= 64 / gsz active lanes leave at
-          // least (gsz - 1) n_act)
-          uint64_t fm = ~act;
-          for (uint32_t i = 0, k = g * (gsz - 1u) + split_lane(gsz) - 1u; i < k; ++i) fm &= fm - 1u;
-          st.init(lds_stack, spill, spill_cap, (threadIdx.x & ~63u) + (uint32_t)__builtin_ctzll(fm));
-          s.cur = kNone;
-        }
+        // lanes 1 .. gsz-1 of group g take the free columns (gsz-1)g ..
+        // (gsz-1)g + gsz-2 in lane order (the n_act <= 64 / gsz active lanes
+        // leave at least (gsz - 1) n_act)
+        if (split && has && split_lane(gsz) != 0u) fresh_helper(~act, g * (gsz - 1u) + split_lane(gsz) - 1u);
         if (COUNT) tl_coop = (uint64_t)wall_clock64();
+      }
+    }
+    // split mode: as rays finish, the remaining ones move into wider groups
+    // (64 / (rays left rounded up to 8, 16, 32) lanes).  A ray's lanes keep
+    // their traversals and stacks (a lane's Stack points at the column it
+    // owns); the new lanes start idle in columns that no remaining ray's lane
+    // owns.  Groups stay at 8 lanes below 8 rays: 16 or 32 lanes per ray ran
+    // slower (more speculative fetches per ray than its tree can use).
+    if (split && gsz < 8u) {
+      const uint64_t heads = __ballot(phase != 0u && split_lane(gsz) == 0u);
+      const uint32_t n = (uint32_t)__popcll(heads);
+      const uint32_t ng = n <= 8u ? 8u : n <= 16u ? 4u : 2u;
+      if (n > 0u && ng > gsz) {
+        const uint32_t col = (uint32_t)(st.lds - (lds_entry*)lds_stack) & 63u;
+        uint64_t used = 0;
+        for (uint32_t b = 0; b < 64u; ++b)
+          if (__ballot(phase != 0u && col == b)) used |= 1ull << b;
+        const uint32_t r = lane_id() / ng, jn = split_lane(ng);
+        uint64_t m = heads;
+        for (uint32_t i = 0; i < r && m; ++i) m &= m - 1u;
+        const bool has = r < n;
+        const uint32_t head = has ? (uint32_t)__builtin_ctzll(m) : lane_id();
+        take_state((int)(has ? head + (jn < gsz ? jn : 0u) : lane_id()));
+        if (!has) phase = 0u, pix_trips = 0u;
+        // new lanes gsz .. ng-1 of group r: free columns r (ng - gsz) + ... in
+        // lane order (n <= 64 / ng remaining rays own n gsz columns)
+        if (has && jn >= gsz) fresh_helper(~used, r * (ng - gsz) + jn - gsz);
+        gsz = ng;
       }
     }
     const bool writer = !coop || (lane_id() & (gsz - 1u)) == 0u;

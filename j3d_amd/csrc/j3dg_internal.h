@@ -225,7 +225,7 @@ struct Ctx {
   hipEvent_t splat_ev[2] = {nullptr, nullptr};  // around each k_splat_depth of j3dg_splat_points
   float splat_ms = 0.f;                          // their sum over the last call's clouds
   bool counting = false;
-  j3dg_tuning tuning{16, 1, 3, 256, 16, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
+  j3dg_tuning tuning{16, 1, 4, 256, 16, 0, 1};  // depth, rays per item, coop tail, block, refill, split, order
   std::vector<Mesh> meshes;
   std::vector<Target> targets;
   std::vector<PointCloud> clouds;

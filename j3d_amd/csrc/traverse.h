@@ -716,7 +716,7 @@ __device__ inline bool coop_step(Trav& s, Hit& best, Stack<D>& st, Counters& c) 
 __device__ inline uint32_t split_lane(uint32_t gsz) { return lane_id() & (gsz - 1u); }
 // bits of the lanes of this lane's group in a wave-wide mask
 __device__ inline uint32_t split_mask(uint64_t m, uint32_t gsz) {
-  return (uint32_t)(m >> (lane_id() & ~(gsz - 1u))) & ((1u << gsz) - 1u);
+  return (uint32_t)((m >> (lane_id() & ~(gsz - 1u))) & ((1ull << gsz) - 1ull));
 }
 
 // Position of the k-th set bit of m (k < popcount(m)).

@@ -10,7 +10,7 @@ their address is taken (hipLaunchKernel finds them by address), and batching
 switched on.  `variant` plants the two mistakes the batch path once had, to show
 the emulator finds what faulted the GPU:
   "wrong_table": the batch launch reads the single-frame object table;
-  "no_frame_handoff": the cooperative tail moves a ray without its frame.
+  "no_frame_handoff": the tail modes move a ray without its frame.
 The allocation-free entry point's fixed array is placed in fenced memory too.
 """
 import os
@@ -53,11 +53,11 @@ def patched_sources(out, variant=None):
     if variant == "wrong_table":
         s = _sub(s, "const DevObject* objs = mode == kModeBatch ? d_bobjs : d_objs;", "const DevObject* objs = d_objs;")
     elif variant == "no_frame_handoff":
-        s = _sub(s, """        if (M == kModeBatch) {  // the ray's frame travels with it
-          shfl_from(fi, leader);
-          shfl_from(lnobj, leader);
-          shfl_from(lobjs, leader);
-        }
+        s = _sub(s, """    if (M == kModeBatch) {  // the ray's frame travels with it
+      shfl_from(fi, src);
+      shfl_from(lnobj, src);
+      shfl_from(lobjs, src);
+    }
 """, "")
     elif variant is not None:
         raise ValueError(variant)
