@@ -274,8 +274,10 @@ int j3dg_render_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames, const uint32
 
 /* ---- autotuner (no counterpart in the reference) -------------------------
    Times the trace stages of `frame` (median of `repeats` frames, default 5)
-   over LDS stack depth {8,16,32} x rays per work item {1,2,4}, applies the
-   fastest tuning to the context and, for a J3DG_LEARNED frame, also picks the
+   over LDS stack depth {8,16,32} x rays per work item {1,2,4}, then the refill
+   threshold {8,16,32,48} and the tail mode (coop_tail) {1,3,4} at the fastest
+   of those, applies the fastest tuning to the context and, for a J3DG_LEARNED
+   frame, also picks the
    fastest bound margin of {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1.6, 3.2}
    (returned; the caller puts it in its frames).  The
    report's width is that of the frame's meshes. */
@@ -295,9 +297,10 @@ int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32_t target, 
 int j3dg_autotune_widths(j3dg_ctx* ctx, const j3dg_frame_desc* frame_w4, const j3dg_frame_desc* frame_w8,
                          uint32_t target, uint32_t repeats, j3dg_autotune_report* report);
 /* A batch (j3dg_render_batch) over BVH width {4,8} x LDS stack depth {8,16,32} x
-   rays per work item {1,2,4}: the n frames given once per width (either list
-   may be NULL), timed as one graph launch (median of `repeats`, default 3);
-   applies the fastest tuning and reports its width. */
+   rays per work item {1,2,4}, then the tail mode {1,3,4} at the fastest: the n
+   frames given once per width (either list may be NULL), timed as one graph
+   launch (median of `repeats`, default 3); applies the fastest tuning and
+   reports its width. */
 int j3dg_autotune_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames_w4, const j3dg_frame_desc* frames_w8,
                         const uint32_t* targets, uint32_t n, uint32_t repeats, j3dg_autotune_report* report);
 

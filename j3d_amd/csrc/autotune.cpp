@@ -4,12 +4,14 @@
 // proposal network, finish + shadow trace; HIP events, median of `repeats`
 // frames after one warm-up) over LDS stack depth {8, 16, 32} x rays per
 // work item {1, 2, 4}, keeps the fastest, then tries the refill threshold
-// {8, 16, 32, 48} there, and for J3DG_LEARNED frames then sweeps the bound
-// margin over {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1.6, 3.2}.  BVH width {4, 8}: the same frame given twice, over width-4 and
-// width-8 uploads of its meshes (the caller owns the uploads), each tuned, the
-// faster kept.  Batches (j3dg_render_batch) are tuned on the time of their
-// one graph launch over width x depth x rays per work item.  The result
-// cache lives with the caller.
+// {8, 16, 32, 48} there, then the tail mode {1, 3, 4} (the cooperative tail;
+// the split tail from 16 or 32 rays left), and for J3DG_LEARNED frames then
+// sweeps the bound margin over {0, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4,
+// 0.8, 1.6, 3.2}.  BVH width {4, 8}: the same frame given twice, over width-4
+// and width-8 uploads of its meshes (the caller owns the uploads), each tuned,
+// the faster kept.  Batches (j3dg_render_batch) are tuned on the time of their
+// one graph launch over width x depth x rays per work item, then the tail
+// mode.  The result cache lives with the caller.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -112,6 +114,23 @@ extern "C" int j3dg_autotune_batch(j3dg_ctx* ctx, const j3dg_frame_desc* frames_
         if (best_ms < 0.f || ms < best_ms) best_ms = ms, best = t, best_width = frame_width(ctx, fs[0]);
       }
   }
+  // the tail mode at the best setting (over the width that setting came from)
+  const j3dg_frame_desc* bfs = best_width == 4 && frames_w4 ? frames_w4 : frames_w8 ? frames_w8 : frames_w4;
+  const uint32_t tails[3] = {1, 3, 4};
+  const j3dg_tuning at = best;
+  for (uint32_t m : tails) {
+    if (m == at.coop_tail) continue;
+    j3dg_tuning t = at;
+    t.coop_tail = m;
+    if ((rc = j3dg_set_tuning(ctx, &t))) return rc;
+    float ms = 0.f;
+    if ((rc = time_batch(ctx, bfs, targets, n, repeats, &ms))) {
+      j3dg_set_tuning(ctx, &base);
+      return rc;
+    }
+    rep->n_tried++;
+    if (ms < best_ms) best_ms = ms, best = t;
+  }
   if ((rc = j3dg_set_tuning(ctx, &best))) return rc;
   rep->best = best;
   rep->best_ms = best_ms;
@@ -154,6 +173,21 @@ extern "C" int j3dg_autotune(j3dg_ctx* ctx, const j3dg_frame_desc* frame, uint32
     if (f == at.refill_min) continue;
     j3dg_tuning t = at;
     t.refill_min = f;
+    if ((rc = j3dg_set_tuning(ctx, &t))) return rc;
+    float ms = 0.f;
+    if ((rc = time_frame(ctx, *frame, target, repeats, &ms))) {
+      j3dg_set_tuning(ctx, &base);
+      return rc;
+    }
+    rep->n_tried++;
+    if (ms < best_ms) best_ms = ms, best = t;
+  }
+  const uint32_t tails[3] = {1, 3, 4};
+  const j3dg_tuning at2 = best;
+  for (uint32_t m : tails) {
+    if (m == at2.coop_tail) continue;
+    j3dg_tuning t = at2;
+    t.coop_tail = m;
     if ((rc = j3dg_set_tuning(ctx, &t))) return rc;
     float ms = 0.f;
     if ((rc = time_frame(ctx, *frame, target, repeats, &ms))) {

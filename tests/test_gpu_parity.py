@@ -381,11 +381,11 @@ def test_autotune_keeps_results(renderer, tmp_path):
     cache = str(tmp_path / "tune.json")
     try:
         rep = renderer.autotune(fr, tg, repeats=2, cache=cache, key="torus@480x270")
-        assert rep["n_tried"] == 12 and not rep["cached"] and rep["width"] == 8
+        assert rep["n_tried"] == 14 and not rep["cached"] and rep["width"] == 8
         tun = renderer.get_tuning()
         assert tun == rep["tuning"]
         assert tun["stack_depth"] in (8, 16, 32) and tun["rays_per_item"] in (1, 2, 4)
-        assert tun["refill_min"] in (8, 16, 32, 48)
+        assert tun["refill_min"] in (8, 16, 32, 48) and tun["coop_tail"] in (1, 3, 4)
         assert rep["best_ms"] <= rep["baseline_ms"]
         g, im = renderer.render(fr, tg)
         renderer.set_tuning(**base)
@@ -401,7 +401,7 @@ def test_autotune_keeps_results(renderer, tmp_path):
 
 def test_autotune_chooses_width(renderer):
     """j3dg_autotune_widths tunes the same frame over width-4 and width-8
-    uploads of its mesh (12 settings each), applies the faster tuning and
+    uploads of its mesh (14 settings each), applies the faster tuning and
     reports its width; the frame at the chosen width matches the oracle."""
     v, t, cam, cs, ci, pivot, diag, light = torus_scene(300, 120, seed=11)
     s4 = renderer.upload_mesh(v, t, db_id=J.make_db_id(1, 0), width=4)
@@ -415,11 +415,11 @@ def test_autotune_chooses_width(renderer):
     base = renderer.get_tuning()
     try:
         rep = renderer.autotune_widths(f4, f8, tg, repeats=2)
-        assert rep["width"] in (4, 8) and rep["n_tried"] == 24
+        assert rep["width"] in (4, 8) and rep["n_tried"] == 28
         assert renderer.get_tuning() == rep["tuning"]
         g, im = renderer.render(f8 if rep["width"] == 8 else f4, tg)
         only8 = renderer.autotune_widths(None, f8, tg, repeats=2)
-        assert only8["width"] == 8 and only8["n_tried"] == 12
+        assert only8["width"] == 8 and only8["n_tried"] == 14
     finally:
         renderer.set_tuning(**base)
         renderer.destroy_target(tg)
