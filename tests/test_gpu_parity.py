@@ -477,3 +477,50 @@ def test_surface_color_arguments(renderer):
     with pytest.raises(J.J3dgError):
         renderer.set_mesh_colors(99, vertex_colors=np.zeros((len(v), 3), np.float32))
     renderer.remove_mesh(slot)
+
+
+def test_tail_modes_give_the_oracles_results(renderer):
+    """Every tail mode -- coop_tail 0 (none), 1 (the cooperative tail), 2-4
+    (the split tail from 8, 16 or 32 rays left, lanes of a group traversing
+    parts of a ray's tree on stacks of their own and regrouping as rays
+    finish) -- at stack depth 8 (spilling) and 16 gives the oracle's frames
+    and hits bit for bit: a 960x540 frame of a 312k-triangle torus with
+    directional shadows, a frame of three posed objects with point-light
+    shadows (a ray moves on from object to object), closest- and any-hit
+    queries over the three."""
+    v, t, cam, cs, ci, pivot, diag, light = torus_scene(600, 260, seed=13)
+    parts = [(v, t, None), J.torus(200, 90, R=0.5, r=0.18, seed=14) + (J.translation_camera(0.9, 0.4, 0.2)[0],),
+             J.icosphere(4, seed=15) + (J.translation_camera(-0.8, -0.3, 0.5)[0],)]
+    slots = [renderer.upload_mesh(a, b, db_id=J.make_db_id(5, k), cs=p) for k, (a, b, p) in enumerate(parts)]
+    meshes = [O.Mesh(a, b, cs=p, db_id=J.make_db_id(5, k)) for k, (a, b, p) in enumerate(parts)]
+    frames = [J.make_frame(960, 540, cam, cs, ci, pivot, diag, [slots[0]], flags=EDGES | ffi.SHADOW | ffi.DIRECTIONAL_LIGHT,
+                           light_dir=light),
+              J.make_frame(640, 360, cam, cs, ci, pivot, diag * 1.6, slots, flags=ffi.SHADING | ffi.SHADOW)]
+    mc, cav = J.make_matcap(0)
+    renderer.set_matcap(mc, cav)
+    ref = [O.render([meshes[0]], frames[0], mc, cav)[:2], O.render(meshes, frames[1], mc, cav)[:2]]
+    rays = random_rays(100000, 21, np.zeros(3), 1.6)
+    ref_hits = O.closest_hits(meshes, rays)
+    ref_any = O.any_hits(meshes, rays)
+    base = renderer.get_tuning()
+    try:
+        for depth in (8, 16):
+            for mode in range(5):
+                renderer.set_tuning(**dict(base, coop_tail=mode, stack_depth=depth))
+                for fr, (og, oim) in zip(frames, ref):
+                    tg = renderer.create_target(fr.width, fr.height)
+                    try:
+                        g, im = renderer.render(fr, tg)
+                    finally:
+                        renderer.destroy_target(tg)
+                    _assert_parity(g, im, og, oim, "tail mode %d, depth %d, %dx%d" % (mode, depth, fr.width, fr.height))
+                got = renderer.trace_rays(slots, rays)
+                for k in ("obj", "tri"):
+                    assert np.array_equal(got[k], ref_hits[k]), (mode, depth, k)
+                for k in ("t", "u", "v"):
+                    assert np.array_equal(got[k].view(np.uint32), ref_hits[k].view(np.uint32)), (mode, depth, k)
+                assert np.array_equal(renderer.any_hit(slots, rays), ref_any), (mode, depth)
+    finally:
+        renderer.set_tuning(**base)
+        for s in slots:
+            renderer.remove_mesh(s)
