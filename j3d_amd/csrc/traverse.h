@@ -775,20 +775,37 @@ __device__ inline bool split_step(Trav& s, Hit& best, Stack<D>& st, Counters& c,
     group_best(best, gsz);
     return true;
   }
-  // the k-th idle lane takes the top entry of the k-th donor
+  // Idle lanes are dealt round-robin over the donors: the k-th idle lane takes
+  // entry k / n_d from the top of donor k mod n_d, reading it from the
+  // donor's LDS column (a stack's top D entries are always there, and no donor
+  // gives more than gsz - 1 < D).  Then each donor drops what it gave, moving
+  // spilled entries back into the freed slots as pop does.
   const uint32_t j = split_lane(gsz), below = (1u << j) - 1u;
-  const uint32_t n = min((uint32_t)__popc(idle), (uint32_t)__popc(donors));
-  uint32_t ref = kNone;
-  float dist = INFINITY;
-  if (busy && st.sp > 0u && (uint32_t)__popc(donors & below) < n) st.pop(ref, dist);
+  const uint32_t n_d = (uint32_t)__popc(donors), n_i = (uint32_t)__popc(idle);
   int src = (int)lane_id();
+  uint32_t k = 0, depth = 0;
   if (!busy) {
-    const uint32_t k = (uint32_t)__popc(idle & below);
-    if (k < n) src = (int)((lane_id() & ~(gsz - 1u)) + nth_set(donors, k));
+    k = (uint32_t)__popc(idle & below);
+    depth = k / n_d;
+    src = (int)((lane_id() & ~(gsz - 1u)) + nth_set(donors, k % n_d));
   }
-  const uint32_t r2 = (uint32_t)__shfl((int)ref, src, 64);
-  const float d2 = __shfl(dist, src, 64);
-  if (!busy && src != (int)lane_id() && d2 <= s.thi) s.cur = r2;
+  const uint32_t dsp = (uint32_t)__shfl((int)st.sp, src, 64);
+  lds_entry* dl = st.lds;
+  shfl_from(dl, src);
+  j3d_u32x2 got = {kNone, 0u};
+  const bool take = !busy && depth < dsp;
+  if (take) got = dl[((dsp - 1u - depth) & (D - 1)) * st.stride];
+  // every receiver has read its entry before any donor refills a slot
+  __builtin_amdgcn_wave_barrier();
+  if (busy && st.sp > 0u) {
+    const uint32_t r = (uint32_t)__popc(donors & below);
+    const uint32_t dealt = n_i > r ? (n_i - r + n_d - 1u) / n_d : 0u;
+    for (uint32_t i = 0; i < min(dealt, st.sp); ++i) {
+      --st.sp;
+      if (st.sp >= (uint32_t)D) st.lds[(st.sp & (D - 1)) * st.stride] = st.spill[(uint64_t)(st.sp - D) * st.sstride];
+    }
+  }
+  if (take && __uint_as_float(got.y) <= s.thi) s.cur = got.x;
   return false;
 }
 
