@@ -37,9 +37,11 @@ enum : uint32_t {
   kEvPrimary = 1u, kEvHit = 2u, kEvShadowed = 4u, kEvResolved = 8u, kEvFallback = 16u, kEvShadowRay = 32u
 };
 
-// Traversal steps a lane may take in one wave loop trip (3 measured best of
-// 1..16 on the 28M-triangle frame: 1 -> 0.87 ms, 2 -> 0.84, 3 -> 0.80, 4 -> 0.82).
-constexpr int kStepsPerTrip = 3;
+// Traversal steps a lane may take in one wave loop trip, measured on the
+// 28M-triangle frame: with the cooperative tail 3 was best (1 -> 0.87 ms,
+// 2 -> 0.84, 3 -> 0.80, 4 -> 0.82); with the split tail 4 is (2 -> 0.723-0.728,
+// 3 -> 0.697-0.707, 4 -> 0.679-0.697, 5 -> 0.708-0.712, 6 -> 0.705-0.707).
+constexpr int kStepsPerTrip = 4;
 // A cooperative group (the tail with at most 8 rays left in the wave) takes up
 // to kCoopSteps steps per trip: no refill shares its trips any more (28M-
 // triangle frame: 1 -> 0.80 ms, 4 -> 0.78; 8 traversal steps per trip for the
